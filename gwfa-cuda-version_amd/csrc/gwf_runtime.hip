@@ -1,0 +1,437 @@
+// gwf_runtime.hip — the C-ABI of libgwfa_hip.so (include/gwfa.h): graph
+// index, device-resident graph and reads, batched alignment with automatic
+// re-runs of reads that outgrow their workspace, and the host-callable gwf_ed.
+#include <algorithm>
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "gwf_dev.h"
+#include "../../include/gwfa.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+void set_err(const char *fmt, ...) {
+	char buf[512];
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(buf, sizeof(buf), fmt, ap);
+	va_end(ap);
+	g_err = buf;
+}
+
+#define HIPCHK(x)                                                                  \
+	do {                                                                           \
+		hipError_t e_ = (x);                                                       \
+		if (e_ != hipSuccess) {                                                    \
+			set_err("%s failed: %s", #x, hipGetErrorString(e_));                   \
+			return GWF_ERR_HIP;                                                    \
+		}                                                                          \
+	} while (0)
+
+constexpr size_t kSeqPad = 2048;  // bytes readable past the end of packed sequences
+
+int device_ready() {
+	int n = 0;
+	if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+		set_err("no HIP device visible");
+		return GWF_ERR_NODEV;
+	}
+	return GWF_OK;
+}
+
+}  // namespace
+
+struct gwf_dgraph_s {
+	DGraph g;
+	void *mem = nullptr;
+	u32 n_vtx = 0;
+};
+
+struct gwf_dreads_s {
+	DReads r;
+	void *mem = nullptr;
+	std::vector<i32> ql, v0;
+};
+
+namespace {
+
+std::mutex g_cache_mu;
+std::unordered_map<const gwf_graph_t *, gwf_dgraph_t *> g_cache;
+
+// Initial workspace: the reference's per-read sizing (main.cu:498-504) plus
+// headroom.  GWF_DEBUG_CAP_A overrides it (tests use it to force the
+// overflow -> re-run path).
+Caps caps_for(i32 ql, int scale) {
+	Caps c;
+	const char *dbg_env = getenv("GWF_DEBUG_CAP_A");
+	const int64_t dbg = dbg_env ? atoll(dbg_env) : 0;
+	int64_t a = dbg > 0 ? dbg : (int64_t)(1.6 * ql) + 256;
+	a *= scale;
+	if (a > (1 << 26)) a = 1 << 26;
+	c.a = (i32)a;
+	c.s = (i32)std::min<int64_t>(3 * a + 8, INT32_MAX / 4);
+	c.x = (i32)a;
+	c.f = (i32)a;
+	c.t = (i32)a;
+	c.h = (i32)a;
+	return c;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *gwf_last_error(void) { return g_err.c_str(); }
+
+// main.cu:246-270: sort arcs by a = src << 32 | dst, aux[v] = first << 32 | count.
+void gwf_ed_index(gwf_graph_t *g) {
+	if (!g) return;
+	free(g->aux);
+	g->aux = (uint64_t *)calloc(g->n_vtx ? g->n_vtx : 1, sizeof(uint64_t));
+	if (!g->aux || g->n_arc == 0) return;
+	std::stable_sort(g->arc, g->arc + g->n_arc, [](const gwf_arc_t &x, const gwf_arc_t &y) { return x.a < y.a; });
+	uint32_t st = 0;
+	for (uint32_t i = 1; i <= g->n_arc; ++i) {
+		if (i == g->n_arc || g->arc[i].a >> 32 != g->arc[st].a >> 32) {
+			uint32_t v = (uint32_t)(g->arc[st].a >> 32);
+			if (v < g->n_vtx) g->aux[v] = (uint64_t)st << 32 | (i - st);
+			st = i;
+		}
+	}
+}
+
+void gwf_cleanup(gwf_graph_t *g) {
+	if (!g) return;
+	{
+		std::lock_guard<std::mutex> lk(g_cache_mu);
+		auto it = g_cache.find(g);
+		if (it != g_cache.end()) {
+			gwf_dgraph_free(it->second);
+			g_cache.erase(it);
+		}
+	}
+	free(g->aux);
+	g->aux = nullptr;
+}
+
+int gwf_dgraph_upload(const gwf_graph_t *g, gwf_dgraph_t **out) {
+	if (!g || !out) { set_err("gwf_dgraph_upload: NULL argument"); return GWF_ERR_ARG; }
+	*out = nullptr;
+	if (!g->aux && g->n_arc) { set_err("gwf_dgraph_upload: graph not indexed (call gwf_ed_index)"); return GWF_ERR_ARG; }
+	if (g->n_vtx == 0 || g->n_vtx >= GWF_MAX_VTX) { set_err("gwf_dgraph_upload: n_vtx %u out of range", g->n_vtx); return GWF_ERR_ARG; }
+	int rc = device_ready();
+	if (rc) return rc;
+	const u32 nv = g->n_vtx, na = g->n_arc;
+	std::vector<uint4> vinfo(nv);
+	std::vector<i32> in_ol(nv, -1);
+	uint64_t total = 0;
+	for (u32 v = 0; v < nv; ++v) {
+		if (g->len[v] == 0 || g->len[v] > (u32)GWF_MAX_VLEN) {
+			set_err("gwf_dgraph_upload: vertex %u has length %u (supported: 1..%d)", v, g->len[v], GWF_MAX_VLEN);
+			return GWF_ERR_ARG;
+		}
+		total += g->len[v];
+	}
+	if (total + kSeqPad >= (1ull << 32)) { set_err("gwf_dgraph_upload: graph sequence exceeds 4 GiB"); return GWF_ERR_ARG; }
+	std::vector<u32> aw(na ? na : 1);
+	std::vector<i32> ao(na ? na : 1);
+	for (u32 i = 0; i < na; ++i) {
+		const u32 u = (u32)(g->arc[i].a >> 32), w = (u32)g->arc[i].a;
+		const i32 o = g->arc[i].o;
+		if (u >= nv || w >= nv || o < 0 || (u32)o >= g->len[w]) {
+			set_err("gwf_dgraph_upload: arc %u (%u->%u, overlap %d) is invalid", i, u, w, o);
+			return GWF_ERR_ARG;
+		}
+		if (in_ol[w] >= 0 && in_ol[w] != o) {
+			set_err("gwf_dgraph_upload: vertex %u has in-arcs with different overlaps (%d, %d); "
+			        "the GPU path needs one overlap per target vertex", w, in_ol[w], o);
+			return GWF_ERR_ARG;
+		}
+		in_ol[w] = o;
+		aw[i] = w, ao[i] = o;
+	}
+	uint64_t so = 0;
+	for (u32 v = 0; v < nv; ++v) {
+		const uint64_t ax = g->aux ? g->aux[v] : 0;
+		vinfo[v] = make_uint4((u32)so, g->len[v], (u32)(ax >> 32), (u32)ax);
+		so += g->len[v];
+	}
+	const size_t b_vi = sizeof(uint4) * nv, b_seq = (size_t)total + kSeqPad, b_aw = 4 * (size_t)aw.size(),
+	             b_ao = 4 * (size_t)ao.size();
+	auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+	const size_t bytes = al(b_vi) + al(b_seq) + al(b_aw) + al(b_ao);
+	std::vector<uint8_t> seqh(b_seq, 0);
+	so = 0;
+	for (u32 v = 0; v < nv; ++v) {
+		memcpy(&seqh[so], g->seq[v], g->len[v]);
+		so += g->len[v];
+	}
+	gwf_dgraph_t *dg = new gwf_dgraph_t();
+	if (hipMalloc(&dg->mem, bytes) != hipSuccess) {
+		delete dg;
+		set_err("gwf_dgraph_upload: hipMalloc(%zu) failed", bytes);
+		return GWF_ERR_NOMEM;
+	}
+	uint8_t *p = (uint8_t *)dg->mem;
+	dg->g.vinfo = (const uint4 *)p;
+	hipError_t e = hipMemcpy(p, vinfo.data(), b_vi, hipMemcpyHostToDevice);
+	p += al(b_vi);
+	dg->g.seq = p;
+	if (e == hipSuccess) e = hipMemcpy(p, seqh.data(), b_seq, hipMemcpyHostToDevice);
+	p += al(b_seq);
+	dg->g.arc_w = (const u32 *)p;
+	if (e == hipSuccess) e = hipMemcpy(p, aw.data(), b_aw, hipMemcpyHostToDevice);
+	p += al(b_aw);
+	dg->g.arc_o = (const i32 *)p;
+	if (e == hipSuccess) e = hipMemcpy(p, ao.data(), b_ao, hipMemcpyHostToDevice);
+	dg->g.n_vtx = nv;
+	dg->n_vtx = nv;
+	if (e != hipSuccess) {
+		set_err("gwf_dgraph_upload: hipMemcpy failed: %s", hipGetErrorString(e));
+		gwf_dgraph_free(dg);
+		return GWF_ERR_HIP;
+	}
+	*out = dg;
+	return GWF_OK;
+}
+
+void gwf_dgraph_free(gwf_dgraph_t *dg) {
+	if (!dg) return;
+	if (dg->mem) (void)hipFree(dg->mem);
+	delete dg;
+}
+
+int gwf_dreads_upload(int32_t n, const char *seqs, const int64_t *off, const int32_t *ql, const int32_t *v0,
+                      gwf_dreads_t **out) {
+	if (!out || n < 0 || (n > 0 && (!seqs || !off || !ql))) { set_err("gwf_dreads_upload: bad argument"); return GWF_ERR_ARG; }
+	*out = nullptr;
+	int rc = device_ready();
+	if (rc) return rc;
+	gwf_dreads_t *dr = new gwf_dreads_t();
+	dr->ql.resize(n), dr->v0.resize(n);
+	std::vector<int64_t> noff(n ? n : 1);
+	int64_t total = 0;
+	for (i32 i = 0; i < n; ++i) {
+		if (ql[i] < 0 || ql[i] > GWF_MAX_QLEN) {
+			delete dr;
+			set_err("gwf_dreads_upload: read %d has length %d (supported: 0..%d)", i, ql[i], GWF_MAX_QLEN);
+			return GWF_ERR_ARG;
+		}
+		dr->ql[i] = ql[i];
+		dr->v0[i] = v0 ? v0[i] : 0;
+		if (dr->v0[i] < 0) { delete dr; set_err("gwf_dreads_upload: read %d has v0 %d", i, dr->v0[i]); return GWF_ERR_ARG; }
+		noff[i] = total;
+		total += ql[i] + 1;
+	}
+	std::vector<uint8_t> buf((size_t)total + kSeqPad, 0);
+	for (i32 i = 0; i < n; ++i) memcpy(&buf[noff[i]], seqs + off[i], ql[i]);
+	auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+	const size_t b_seq = buf.size(), b_off = 8 * noff.size(), b_ql = 4 * (size_t)(n ? n : 1);
+	const size_t bytes = al(b_seq) + al(b_off) + 2 * al(b_ql);
+	if (hipMalloc(&dr->mem, bytes) != hipSuccess) {
+		delete dr;
+		set_err("gwf_dreads_upload: hipMalloc(%zu) failed", bytes);
+		return GWF_ERR_NOMEM;
+	}
+	uint8_t *p = (uint8_t *)dr->mem;
+	hipError_t e = hipMemcpy(p, buf.data(), b_seq, hipMemcpyHostToDevice);
+	dr->r.seq = p;
+	p += al(b_seq);
+	if (e == hipSuccess) e = hipMemcpy(p, noff.data(), b_off, hipMemcpyHostToDevice);
+	dr->r.off = (const int64_t *)p;
+	p += al(b_off);
+	if (e == hipSuccess && n) e = hipMemcpy(p, dr->ql.data(), 4 * (size_t)n, hipMemcpyHostToDevice);
+	dr->r.ql = (const i32 *)p;
+	p += al(b_ql);
+	if (e == hipSuccess && n) e = hipMemcpy(p, dr->v0.data(), 4 * (size_t)n, hipMemcpyHostToDevice);
+	dr->r.v0 = (const i32 *)p;
+	dr->r.n = n;
+	if (e != hipSuccess) {
+		set_err("gwf_dreads_upload: hipMemcpy failed: %s", hipGetErrorString(e));
+		gwf_dreads_free(dr);
+		return GWF_ERR_HIP;
+	}
+	*out = dr;
+	return GWF_OK;
+}
+
+void gwf_dreads_free(gwf_dreads_t *dr) {
+	if (!dr) return;
+	if (dr->mem) (void)hipFree(dr->mem);
+	delete dr;
+}
+
+int gwf_ed_run(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t v1, uint32_t max_lag, int32_t kernel, int32_t *out,
+               gwf_run_stats_t *stats) {
+	(void)max_lag;  // accepted and ignored, as in the reference (gwf-ed.cu:1286-1287)
+	const auto t0 = std::chrono::steady_clock::now();
+	if (!dg || !dr || (!out && dr->r.n > 0)) { set_err("gwf_ed_run: NULL argument"); return GWF_ERR_ARG; }
+	if (v1 >= 0) { set_err("gwf_ed_run: v1 >= 0 is not supported on the GPU path"); return GWF_ERR_ARG; }
+	if (kernel != GWF_KERNEL_PLAIN) { set_err("gwf_ed_run: kernel %d is not available", kernel); return GWF_ERR_ARG; }
+	const i32 n = dr->r.n;
+	for (i32 i = 0; i < n; ++i)
+		if ((u32)dr->v0[i] >= dg->n_vtx) { set_err("gwf_ed_run: read %d has v0 %d >= n_vtx %u", i, dr->v0[i], dg->n_vtx); return GWF_ERR_ARG; }
+	gwf_run_stats_t st;
+	memset(&st, 0, sizeof(st));
+	if (n == 0) { if (stats) *stats = st; return GWF_OK; }
+
+	i32 *d_ed = nullptr, *d_status = nullptr, *d_idx = nullptr;
+	int64_t *d_cnt = nullptr;
+	u64 *scratch = nullptr;
+	hipStream_t s = nullptr;
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	int rc = GWF_OK;
+	std::vector<i32> status(n), idx;
+	std::vector<int64_t> cnt(3 * (size_t)n);
+	size_t free_b = 0, tot_b = 0;
+	size_t budget, scratch_bytes = 0;
+	std::vector<i32> todo(n);
+	for (i32 i = 0; i < n; ++i) todo[i] = i;
+	int scale = 1;
+
+#define RUNCHK(x)                                                                  \
+	do {                                                                           \
+		hipError_t e_ = (x);                                                       \
+		if (e_ != hipSuccess) {                                                    \
+			set_err("%s failed: %s", #x, hipGetErrorString(e_));                   \
+			rc = GWF_ERR_HIP;                                                      \
+			goto done;                                                             \
+		}                                                                          \
+	} while (0)
+
+	RUNCHK(hipStreamCreate(&s));
+	RUNCHK(hipEventCreate(&e0));
+	RUNCHK(hipEventCreate(&e1));
+	RUNCHK(hipMalloc(&d_ed, 4 * (size_t)n));
+	RUNCHK(hipMalloc(&d_status, 4 * (size_t)n));
+	RUNCHK(hipMalloc(&d_idx, 4 * (size_t)n));
+	RUNCHK(hipMalloc(&d_cnt, 24 * (size_t)n));
+	RUNCHK(hipMemGetInfo(&free_b, &tot_b));
+	budget = std::min<size_t>(free_b / 2, (size_t)32 << 30);
+
+	while (!todo.empty()) {
+		// batches of consecutive pending reads whose workspaces fit the budget
+		size_t pos = 0;
+		while (pos < todo.size()) {
+			i32 maxql = 0;
+			size_t end = pos, ws = 0;
+			while (end < todo.size()) {
+				const i32 q = std::max(maxql, dr->ql[todo[end]]);
+				const size_t wsb = gwf_ws_bytes(caps_for(q, scale));
+				if (end > pos && wsb * (end - pos + 1) > budget) break;
+				maxql = q, ws = wsb, ++end;
+			}
+			const Caps caps = caps_for(maxql, scale);
+			const i32 nb = (i32)(end - pos);
+			if (ws * nb > scratch_bytes) {
+				if (scratch) (void)hipFree(scratch);
+				scratch = nullptr;
+				scratch_bytes = ws * nb;
+				RUNCHK(hipMalloc(&scratch, scratch_bytes));
+			}
+			RUNCHK(hipMemcpyAsync(d_idx, todo.data() + pos, 4 * (size_t)nb, hipMemcpyHostToDevice, s));
+			RUNCHK(hipEventRecord(e0, s));
+			RUNCHK(gwf_launch_plain(dg->g, dr->r, d_idx, nb, caps, scratch, ws, d_ed, d_status, d_cnt, s));
+			RUNCHK(hipEventRecord(e1, s));
+			RUNCHK(hipEventSynchronize(e1));
+			float ms = 0;
+			RUNCHK(hipEventElapsedTime(&ms, e0, e1));
+			st.kernel_ms += ms;
+			st.n_launch += 1;
+			pos = end;
+		}
+		RUNCHK(hipMemcpy(status.data(), d_status, 4 * (size_t)n, hipMemcpyDeviceToHost));
+		std::vector<i32> again;
+		for (i32 r : todo) {
+			if (status[r] == GWF_ST_OVERFLOW) again.push_back(r);
+			else if (status[r] != GWF_ST_OK) {
+				set_err("gwf_ed_run: read %d ended with internal status %d", r, status[r]);
+				rc = GWF_ERR_STATE;
+				goto done;
+			}
+		}
+		if (!again.empty()) {
+			st.n_retry += (int64_t)again.size();
+			scale *= 4;
+			if (caps_for(1, scale).a >= (1 << 26) && scale > 1 << 12) {
+				set_err("gwf_ed_run: %zu reads exceed the maximum workspace", again.size());
+				rc = GWF_ERR_CAPACITY;
+				goto done;
+			}
+		}
+		todo.swap(again);
+	}
+	RUNCHK(hipMemcpy(out, d_ed, 4 * (size_t)n, hipMemcpyDeviceToHost));
+	RUNCHK(hipMemcpy(cnt.data(), d_cnt, 24 * (size_t)n, hipMemcpyDeviceToHost));
+	for (i32 i = 0; i < n; ++i) {
+		st.sum_steps += cnt[3 * (size_t)i];
+		st.sum_states += cnt[3 * (size_t)i + 1];
+		st.sum_chars += cnt[3 * (size_t)i + 2];
+	}
+done:
+#undef RUNCHK
+	if (scratch) (void)hipFree(scratch);
+	if (d_ed) (void)hipFree(d_ed);
+	if (d_status) (void)hipFree(d_status);
+	if (d_idx) (void)hipFree(d_idx);
+	if (d_cnt) (void)hipFree(d_cnt);
+	if (e0) (void)hipEventDestroy(e0);
+	if (e1) (void)hipEventDestroy(e1);
+	if (s) (void)hipStreamDestroy(s);
+	st.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+	if (stats) *stats = st;
+	return rc;
+}
+
+int gwf_ed_batch(gwf_dgraph_t *dg, int32_t n, const char *const *q, const int32_t *ql, const int32_t *v0, int32_t v1,
+                 uint32_t max_lag, int32_t kernel, int32_t *out, gwf_run_stats_t *stats) {
+	if (n < 0 || (n > 0 && (!q || !ql))) { set_err("gwf_ed_batch: bad argument"); return GWF_ERR_ARG; }
+	std::string buf;
+	std::vector<int64_t> off(n ? n : 1);
+	for (i32 i = 0; i < n; ++i) {
+		if (ql[i] < 0) { set_err("gwf_ed_batch: negative length"); return GWF_ERR_ARG; }
+		off[i] = (int64_t)buf.size();
+		buf.append(q[i], (size_t)ql[i]);
+		buf.push_back('\0');
+	}
+	gwf_dreads_t *dr = nullptr;
+	int rc = gwf_dreads_upload(n, buf.data(), off.data(), ql, v0, &dr);
+	if (rc) return rc;
+	rc = gwf_ed_run(dg, dr, v1, max_lag, kernel, out, stats);
+	gwf_dreads_free(dr);
+	return rc;
+}
+
+int32_t gwf_ed(const gwf_graph_t *g, int32_t ql, const char *q, int32_t v0, int32_t v1, uint32_t max_lag,
+               gwf_edbuf_t *buf, uint64_t *diag_shared) {
+	(void)buf;
+	(void)diag_shared;
+	if (!g || (!q && ql > 0) || ql < 0) { set_err("gwf_ed: bad argument"); return GWF_ERR_ARG; }
+	if (v0 < 0 || (uint32_t)v0 >= g->n_vtx) { set_err("gwf_ed: v0 %d out of range", v0); return GWF_ERR_ARG; }
+	gwf_dgraph_t *dg = nullptr;
+	{
+		std::lock_guard<std::mutex> lk(g_cache_mu);
+		auto it = g_cache.find(g);
+		if (it != g_cache.end()) dg = it->second;
+		else {
+			int rc = gwf_dgraph_upload(g, &dg);
+			if (rc) return rc;
+			g_cache.emplace(g, dg);
+		}
+	}
+	const char *qq = q ? q : "";
+	int32_t out = -1;
+	int rc = gwf_ed_batch(dg, 1, &qq, &ql, &v0, v1, max_lag, GWF_KERNEL_PLAIN, &out, nullptr);
+	return rc ? rc : out;
+}
+
+}  // extern "C"

@@ -1,0 +1,191 @@
+/*
+ * gwfa.h — drop-in public header of the MI355X GWFA graph edit-distance library
+ * (libgwfa_hip.so).
+ *
+ * The types and the first three entry points keep the names, field order and
+ * memory layout of the reference header (gwfa.h:6-95), so C/C++ host code
+ * written against the reference compiles and links unchanged.  Differences:
+ *   - the header is plain C99/C++11: no __device__, __align__ or u_int64_t;
+ *   - gwf_ed() is host-callable and runs on the GPU (the reference declares
+ *     it __device__, gwfa.h:95); `buf` and `diag_shared` may be NULL;
+ *   - batch, device-graph, device-read-set and serving entry points are added
+ *     (they have no counterpart in the reference, whose batching lives inline
+ *     in main.cu:454-735).
+ * All arithmetic is exact integer.  Every entry point that can fail returns a
+ * GWF_* status code; nothing is dropped silently.
+ */
+#ifndef GWFA_H
+#define GWFA_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+#define GWF_RESTRICT __restrict
+#else
+#define GWF_RESTRICT restrict
+#endif
+#define GWF_ALIGN16 __attribute__((aligned(16)))
+
+/* ---- types: identical layout to the reference (gwfa.h:6-80) ------------- */
+
+typedef struct {          /* gwfa.h:6-9   16 B */
+	uint64_t a;           /* source vertex << 32 | target vertex */
+	int32_t o;            /* overlap: first offset used in the target */
+} gwf_arc_t;
+
+typedef struct {          /* gwfa.h:11-18 48 B */
+	uint32_t n_vtx, n_arc;
+	uint32_t *len;        /* vertex lengths */
+	uint32_t *src;        /* oriented GFA segment id of each vertex */
+	char **seq;           /* NUL-terminated vertex sequences */
+	gwf_arc_t *arc;       /* arcs, sorted by `a` after gwf_ed_index() */
+	uint64_t *aux;        /* CSR index: aux[v] = first_arc << 32 | n_arcs */
+} gwf_graph_t;
+
+typedef struct {          /* gwfa.h:20-25 */
+	uint64_t *GWF_RESTRICT vd;
+	int32_t *GWF_RESTRICT k;
+	uint32_t *GWF_RESTRICT xo;
+} gwf_diag_soa_t;
+
+typedef struct {          /* gwfa.h:27-31 16 B */
+	uint64_t vd;          /* vertex << 32 | (diagonal + 0x40000000) */
+	int32_t k;            /* last matched offset in the vertex (-1: none) */
+	uint32_t xo;          /* anti-diagonal << 1 | out-of-order flag */
+} gwf_diag_t;
+
+typedef struct {          /* gwfa.h:33-36 16 B */
+	uint64_t vd0;
+	uint64_t vd1;
+} gwf_intv_t;
+
+typedef struct GWF_ALIGN16 { /* gwfa.h:38-41 */
+	uint64_t *GWF_RESTRICT vd0;
+	uint64_t *GWF_RESTRICT vd1;
+} gwf_intv_soa_t;
+
+typedef struct {          /* gwfa.h:44-52 28 B */
+	int32_t n_intv;
+	int32_t n_tmp;
+	int32_t n_swap;
+	int32_t n_ooo;
+	int32_t n_A;
+	int32_t n_B;
+	int32_t n_a;
+} counters;
+
+typedef struct {          /* gwfa.h:54-80 192 B; accepted and ignored by gwf_ed() */
+	gwf_intv_soa_t intv;
+	int32_t max_intv;
+	gwf_intv_soa_t tmp;
+	int32_t max_tmp;
+	gwf_intv_soa_t swap;
+	int32_t max_swap;
+	gwf_diag_t *ooo;
+	int32_t max_ooo;
+	gwf_diag_soa_t a_buf;
+	int32_t max_a;
+	gwf_diag_t *B;
+	int32_t max_B;
+	gwf_diag_t *GWF_RESTRICT A;
+	int32_t max_A;
+	int32_t head_A;
+	int32_t tail_A;
+	uint64_t *GWF_RESTRICT ha;
+	uint32_t ha_mask;
+} gwf_edbuf_t;
+
+/* ---- status codes -------------------------------------------------------- */
+
+/* -1 is never an error code: it is gwf_ed()'s "no end reachable" result. */
+#define GWF_OK             0
+#define GWF_ERR_ARG       -2   /* invalid argument (NULL pointer, bad vertex id, ...) */
+#define GWF_ERR_HIP       -3   /* HIP runtime error; see gwf_last_error() */
+#define GWF_ERR_NOMEM     -4   /* host or device allocation failed */
+#define GWF_ERR_NODEV     -5   /* no HIP device visible */
+#define GWF_ERR_CAPACITY  -6   /* a read needs more wavefront state than the hard cap */
+#define GWF_ERR_STATE     -7   /* call not valid in the current state (e.g. server stopped) */
+#define GWF_ERR_BUDGET    -8   /* a request can never fit the serving state budget */
+
+/* ED result value of a read whose alignment has no end (only when v1 >= 0). */
+#define GWF_ED_NONE       -1
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- reference entry points (gwfa.h:87-88, 95) --------------------------- */
+
+/* Build the CSR arc index: sorts g->arc by `a` and allocates g->aux
+ * (main.cu:242-270).  Host only; not thread-safe on the same graph. */
+void gwf_ed_index(gwf_graph_t *g);
+
+/* Free g->aux (main.cu:272-277) and drop any device copy of g cached by
+ * gwf_ed(). */
+void gwf_cleanup(gwf_graph_t *g);
+
+/* Edit distance of query q[0..ql) against g, starting at offset 0 of vertex
+ * v0 and ending anywhere (v1 < 0) or at the last base of vertex v1 (v1 >= 0).
+ * Same arguments and result as the reference device routine (gwfa.h:95,
+ * gwf-ed.cu:1292-1337): the ED, or -1 when v1 >= 0 is unreachable.
+ * max_lag is accepted and ignored, as in the reference (gwf-ed.cu:1286-1287).
+ * buf and diag_shared may be NULL (they are ignored: state is library-owned).
+ * Host-callable; runs on the current HIP device.  The device copy of g is
+ * cached per graph pointer until gwf_cleanup(g).  Returns a negative GWF_ERR_*
+ * code (< -1) on failure. */
+int32_t gwf_ed(const gwf_graph_t *g, int32_t ql, const char *q, int32_t v0, int32_t v1,
+               uint32_t max_lag, gwf_edbuf_t *buf, uint64_t *diag_shared);
+
+/* ---- additions: device-resident graph and reads, batch alignment ---------- */
+/* These have no counterpart in the reference API; they replace its inline
+ * batch runtime (main.cu:417-452 graph upload, main.cu:454-735 stream loop). */
+
+typedef struct gwf_dgraph_s gwf_dgraph_t;   /* packed CSR graph resident in HBM */
+typedef struct gwf_dreads_s gwf_dreads_t;   /* packed read set resident in HBM */
+
+/* Kernel selection for gwf_ed_batch / gwf_ed_run. */
+#define GWF_KERNEL_PLAIN       0  /* one 64-lane wavefront per read, grid = reads */
+#define GWF_KERNEL_PERSISTENT  1  /* persistent workgroups, cost-ordered read queue */
+
+typedef struct {
+	double   kernel_ms;        /* device time of the alignment launches (HIP events) */
+	double   wall_ms;          /* host wall time of the call */
+	int64_t  n_launch;         /* alignment kernel launches */
+	int64_t  n_retry;          /* reads re-run with a larger state capacity */
+	int64_t  sum_steps;        /* sum over reads of score steps (ED + 1) */
+	int64_t  sum_states;       /* sum over reads and steps of wavefront size */
+	int64_t  sum_chars;        /* characters compared by extension */
+	int64_t  n_spill;          /* reads whose state left LDS for HBM */
+} gwf_run_stats_t;
+
+/* Upload an indexed graph (gwf_ed_index() done) to the current device. */
+int gwf_dgraph_upload(const gwf_graph_t *g, gwf_dgraph_t **out);
+void gwf_dgraph_free(gwf_dgraph_t *dg);
+
+/* Upload n reads: seqs + off[i] holds read i of length ql[i]; v0[i] is its
+ * start vertex (NULL: all 0). */
+int gwf_dreads_upload(int32_t n, const char *seqs, const int64_t *off, const int32_t *ql,
+                      const int32_t *v0, gwf_dreads_t **out);
+void gwf_dreads_free(gwf_dreads_t *dr);
+
+/* Align every read of dr against dg; out[i] (host memory) receives read i's
+ * ED.  stats may be NULL.  Reads whose state outgrows the working capacity
+ * are re-run with a larger one; GWF_ERR_CAPACITY only past the hard cap. */
+int gwf_ed_run(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t v1, uint32_t max_lag,
+               int32_t kernel, int32_t *out, gwf_run_stats_t *stats);
+
+/* Convenience: upload reads from host pointers, run, free.  q[i] points at
+ * read i (ql[i] bytes); v0 may be NULL. */
+int gwf_ed_batch(gwf_dgraph_t *dg, int32_t n, const char *const *q, const int32_t *ql,
+                 const int32_t *v0, int32_t v1, uint32_t max_lag, int32_t kernel,
+                 int32_t *out, gwf_run_stats_t *stats);
+
+/* Text of the last error seen by this thread ("" if none). */
+const char *gwf_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GWFA_H */

@@ -1,0 +1,206 @@
+"""Pin the oracle (the CPU restatement of gwf_ed, gwf-ed.cu:80-1337) before
+trusting it: against the independent brute-force DP on random graphs,
+against closed forms, invariants and the reference's fixture inputs."""
+import os
+import random
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, read_ints
+
+
+def rand_graph(gw, rng, nv, dag=True, ovl=False, maxlen=12):
+    seqs = []
+    for _ in range(nv):
+        L = rng.randint(1, 3 if rng.random() < 0.5 else maxlen)
+        alph = "ACGT"[: rng.randint(2, 4)]
+        seqs.append("".join(rng.choice(alph) for _ in range(L)))
+    fixed_o = [((w * 7 + 3) % len(seqs[w])) if ovl else 0 for w in range(nv)]
+    arcs = []
+    for v in range(nv):
+        for _ in range(rng.randint(0, 2)):
+            if dag:
+                if v + 1 >= nv:
+                    break
+                w = rng.randint(v + 1, nv - 1)
+            else:
+                w = rng.randrange(nv)
+            arcs.append((v, w, fixed_o[w]))
+    g = gw.Graph.from_lists(seqs, arcs)
+    g.index()
+    return g, seqs
+
+
+def rand_read(rng, n):
+    return "".join(rng.choice("ACGT") for _ in range(n))
+
+
+@pytest.mark.parametrize("dag,ovl", [(True, False), (False, False), (True, True), (False, True)])
+def test_oracle_equals_dp_random(gw, orc, dag, ovl):
+    """Exact ED for v1 < 0 when all in-arcs of a vertex share one overlap."""
+    rng = random.Random(hash((dag, ovl)) & 0xffff)
+    for it in range(600):
+        g, seqs = rand_graph(gw, rng, rng.randint(1, 12), dag, ovl)
+        q = rand_read(rng, rng.randint(0, 30))
+        assert orc.ed(g, q) == orc.dp(g, q), (it, seqs, g.arcs(), q)
+
+
+def test_oracle_equals_dp_larger(gw, orc):
+    rng = random.Random(7)
+    for it in range(40):
+        g, seqs = rand_graph(gw, rng, rng.randint(20, 60), True, False, maxlen=40)
+        q = rand_read(rng, rng.randint(50, 300))
+        assert orc.ed(g, q) == orc.dp(g, q), it
+
+
+def test_dp_catches_planted_mistakes(gw, orc):
+    """The DP pins plausible mistakes: an oracle that charged 2 per
+    substitution, or ignored the free end, would disagree on these."""
+    g = gw.Graph.from_lists(["ACGTACGTACGT"], [])
+    g.index()
+    assert orc.dp(g, "ACGAACGT") == 1 == orc.ed(g, "ACGAACGT")      # one substitution
+    assert orc.dp(g, "ACGT") == 0 == orc.ed(g, "ACGT")              # free end in the graph
+    assert orc.dp(g, "ACGTTACGT") == 1 == orc.ed(g, "ACGTTACGT")    # one insertion
+    assert orc.dp(g, "ACGACGT") == 1 == orc.ed(g, "ACGACGT")        # one deletion
+    assert orc.dp(g, "TACGT") == 1 == orc.ed(g, "TACGT")            # leading insertion
+
+
+def test_v1_target_vertex_is_upper_bound(gw, orc):
+    """For v1 >= 0 the reference algorithm skips the insertion of a vertex-end
+    state when every out-arc matches (gwf-ed.cu:1225); its result is then an
+    upper bound of the exact end-at-v1 distance, never below it."""
+    rng = random.Random(11)
+    for _ in range(300):
+        g, _ = rand_graph(gw, rng, rng.randint(1, 10))
+        q = rand_read(rng, rng.randint(0, 20))
+        v1 = rng.randrange(g.n_vtx)
+        a, b = orc.ed(g, q, v1=v1), orc.dp(g, q, v1=v1)
+        assert (a == -1) == (b == -1)
+        assert a >= b
+
+
+def sellers(t, q):
+    """Semi-global edit distance: q fully aligned to a prefix of t."""
+    prev = np.arange(len(q) + 1)
+    best = prev[-1]
+    for c in t:
+        cur = np.empty_like(prev)
+        cur[0] = prev[0] + 1
+        for j in range(1, len(q) + 1):
+            cur[j] = min(prev[j] + 1, cur[j - 1] + 1, prev[j - 1] + (q[j - 1] != c))
+        prev = cur
+        best = min(best, prev[-1])
+    return int(best)
+
+
+def test_closed_forms(gw, orc):
+    rng = random.Random(3)
+    t = rand_read(rng, 200)
+    g = gw.Graph.from_lists([t], [])
+    g.index()
+    assert orc.ed(g, t[:137]) == 0                     # exact prefix
+    assert orc.ed(g, "") == 0                          # empty read
+    for _ in range(30):                                # linear graph = Sellers DP
+        q = rand_read(rng, rng.randint(1, 60))
+        assert orc.ed(g, q) == sellers(t, q)
+    q = list(t[:150])                                  # k separated substitutions
+    for p in (10, 40, 70, 100, 130):
+        q[p] = "A" if q[p] != "A" else "C"
+    assert orc.ed(g, "".join(q)) == 5
+    for _ in range(20):                                # ED <= ql
+        q = rand_read(rng, rng.randint(0, 50))
+        assert orc.ed(g, q) <= len(q)
+
+
+def test_all_insertion_end(gw, orc):
+    """ED == ql through an end with offset -1 (the reference loop would spin)."""
+    g = gw.Graph.from_lists(["A", "AAAA"], [(0, 1)])
+    g.index()
+    assert orc.ed(g, "TT") == 2
+    assert orc.ed(g, "TTTTTTTT") == 8
+
+
+def test_short_start_vertex(gw, orc):
+    """A start segment shorter than 32 bp with an exact-prefix read gives 0
+    (the reference's unsigned max_k would skip that extension)."""
+    g = gw.Graph.from_lists(["ACGTACGTAC", "GGTTAACC"], [(0, 1)])
+    g.index()
+    assert orc.ed(g, "ACGTACGTACGGTTAA") == 0
+
+
+def test_invariants(gw, orc):
+    rng = random.Random(5)
+    for _ in range(60):
+        g, seqs = rand_graph(gw, rng, rng.randint(2, 10))
+        arcs = g.arcs()
+        q = rand_read(rng, rng.randint(0, 25))
+        ed = orc.ed(g, q)
+        # relabel vertices (start stays 0)
+        perm = [0] + rng.sample(range(1, len(seqs)), len(seqs) - 1)
+        inv = {old: new for new, old in enumerate(perm)}
+        g2 = gw.Graph.from_lists([seqs[o] for o in perm], [(inv[a], inv[b], o) for a, b, o in arcs])
+        g2.index()
+        assert orc.ed(g2, q) == ed
+        # start index given through v0 instead of being vertex 0
+        rot = list(range(1, len(seqs))) + [0]
+        inv2 = {old: new for new, old in enumerate(rot)}
+        g3 = gw.Graph.from_lists([seqs[o] for o in rot], [(inv2[a], inv2[b], o) for a, b, o in arcs])
+        g3.index()
+        assert orc.ed(g3, q, v0=inv2[0]) == ed
+        # unreachable vertices change nothing
+        g4 = gw.Graph.from_lists(seqs + ["ACGTTT", "GG"], arcs + [(len(seqs), len(seqs) + 1, 0)])
+        g4.index()
+        assert orc.ed(g4, q) == ed
+        # split a vertex into a chain of two
+        v = rng.randrange(len(seqs))
+        if len(seqs[v]) >= 2:
+            cut = rng.randint(1, len(seqs[v]) - 1)
+            s5 = list(seqs) + [seqs[v][cut:]]
+            s5[v] = seqs[v][:cut]
+            nw = len(seqs)
+            a5 = [(nw if a == v else a, b, o) for a, b, o in arcs] + [(v, nw, 0)]
+            g5 = gw.Graph.from_lists(s5, a5)
+            g5.index()
+            assert orc.ed(g5, q) == ed
+
+
+def test_golden_fixture_1000_reads(gw, orc):
+    g = gw.Graph.from_gfa(os.path.join(GOLDEN, "graph_1000.gfa")).index()
+    r = gw.Reads.from_fasta(os.path.join(GOLDEN, "reads_1000.fa.gz"))
+    want = read_ints(os.path.join(GOLDEN, "reads_1000.ed"))
+    got, _, _ = orc.batch(g, r)
+    assert got.tolist() == want
+    for i in range(0, 1000, 97):            # re-check a sample against the DP
+        assert orc.dp(g, r.read(i)) == want[i]
+
+
+def test_golden_fixture_grafo(gw, orc):
+    g = gw.Graph.from_gfa(os.path.join(GOLDEN, "grafo_test.gfa")).index()
+    r = gw.Reads.from_fasta(os.path.join(GOLDEN, "grafo_reads.fa"))
+    want = read_ints(os.path.join(GOLDEN, "grafo_reads.ed"))
+    got, _, _ = orc.batch(g, r)
+    assert got.tolist() == want
+
+
+def test_index_matches_reference_restatement(gw, orc):
+    """gwf_ed_index (product) vs the oracle's restatement of main.cu:246-270."""
+    import ctypes as C
+    rng = random.Random(9)
+    for _ in range(50):
+        g, seqs = rand_graph(gw, rng, rng.randint(1, 30), dag=False)
+        arcs = g.arcs()
+        h = gw.Graph.from_lists(seqs, arcs)
+        aux = np.zeros(max(h.n_vtx, 1), dtype=np.uint64)
+        orc.lib().oracle_ed_index(C.cast(h.ptr, C.c_void_p), aux.ctypes.data)
+        assert g.aux().tolist() == aux[:h.n_vtx].tolist()
+        assert [a[:2] for a in g.arcs()] == [a[:2] for a in h.arcs()]
+
+
+def test_batch_runner_threads(gw, orc):
+    g, r = gw.make_workload(1, 200)
+    g.index()
+    a, st1, _ = orc.batch(g, r, threads=1)
+    b, st8, t = orc.batch(g, r, threads=4)
+    assert a.tolist() == b.tolist()
+    assert st1["states"] == st8["states"] and st1["steps"] == int(a.sum()) + len(a)
