@@ -1,0 +1,153 @@
+#!/usr/bin/env python3
+"""Throughput benchmark: aligned query bases/s of GWFA graph edit distance
+on one MI355X, with every ED checked bit-exact on the CPU-baseline sample.
+
+A step is one pass of the alignment over the whole read set of the workload
+(default: 20,000 x 5 kb reads, errors 1/5/10 % in equal thirds, against a
+1 Mb-backbone bubble graph, every read starting at offset 0 of a sampled
+vertex).  Graph and reads are resident in HBM before the timed region; the
+timed region is K calls of gwf_ed_run (launches + the 4 B/read ED copy-back).
+
+Prints ONE JSON line.  Multi-GPU runs are not part of this build: --gpus N
+with N > 1 exits with a message.
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+METRIC = "aligned query bases/sec on 1 MI355X (bit-exact ED); p50/p99 read latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def load_package():
+    pdir = os.path.join(ROOT, "gwfa-cuda-version_amd")
+    spec = importlib.util.spec_from_file_location("gwfa_amd", os.path.join(pdir, "__init__.py"),
+                                                  submodule_search_locations=[pdir])
+    m = importlib.util.module_from_spec(spec)
+    sys.modules["gwfa_amd"] = m
+    spec.loader.exec_module(m)
+    return m
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(gw, graph, reads, sample, gpu_eds):
+    """Oracle (CPU port of gwf_ed) on the first `sample` reads, all host
+    threads up to 16; also the parity check of those reads."""
+    sys.path.insert(0, ROOT)
+    import oracle
+    idx = np.arange(min(sample, len(reads)))
+    sub = reads.subset(idx)
+    threads = max(1, min(16, os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    eds, st, used = oracle.batch(graph, sub, threads=threads)
+    dt = time.perf_counter() - t0
+    mism = int((eds != gpu_eds[idx]).sum())
+    return {
+        "value": sub.bases() / dt,
+        "unit": "bases/s",
+        "cores": int(used),
+        "kind": "port",
+        "sample": f"first {len(sub)} reads of the workload ({sub.bases()} bases), "
+                  f"{dt:.2f} s wall on {used} threads",
+    }, {"reads": len(sub), "mismatches": mism}, st
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=2, help="workload 1-4 (default 2)")
+    ap.add_argument("--kernel", choices=["plain", "persistent"], default="plain")
+    ap.add_argument("--reads", type=int, default=None, help="use only the first N reads")
+    ap.add_argument("--cpu-sample", type=int, default=3000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--traffic", default=None, help="JSON with fetch/write bytes per launch from rocprofv3 --pmc")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU only (reads shard independently; "
+              "no multi-GPU path is part of it). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    gw = load_package()
+    kernel = gw.KERNEL_PLAIN if args.kernel == "plain" else gw.KERNEL_PERSISTENT
+    cfg = gw.CONFIGS[args.config]
+    t = time.perf_counter()
+    graph, reads = gw.make_workload(args.config, args.reads)
+    graph.index()
+    log(f"workload {cfg['name']}: {graph.n_vtx} vertices, {graph.n_arc} arcs, {len(reads)} reads, "
+        f"{reads.bases()} bases ({time.perf_counter() - t:.1f} s to generate)")
+    dg, dr = gw.DeviceGraph(graph), gw.DeviceReads(reads)
+
+    for i in range(args.warmup):
+        eds0, st = gw.ed_run(dg, dr, kernel=kernel)
+        log(f"warmup {i}: kernel {st['kernel_ms']:.1f} ms, wall {st['wall_ms']:.1f} ms")
+    runs = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        eds, st = gw.ed_run(dg, dr, kernel=kernel, timed=True)
+        runs.append((eds, st))
+        log(f"step {i}: kernel {st['kernel_ms']:.1f} ms, wall {st['wall_ms']:.1f} ms, launches {st['n_launch']}, "
+            f"retries {st['n_retry']}")
+    elapsed = time.perf_counter() - t0
+    for e, _ in runs[1:]:
+        if not np.array_equal(e, runs[0][0]):
+            raise SystemExit("bench.py: EDs differ between timed steps")
+    eds, st = runs[-1]
+
+    bases = reads.bases()
+    value = bases * args.steps / elapsed
+    # Algorithmic bytes (per read: ql + 4 + 32 W + 2 C, W = wavefront states
+    # entering each score step, C = characters compared by extension).
+    alg_bytes = bases + 4 * len(reads) + 32 * st["sum_states"] + 2 * st["sum_chars"]
+    kern_s = np.mean([s["kernel_ms"] for _, s in runs]) / 1e3
+    achieved = alg_bytes / kern_s / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+            "kernel": "gwf_plain_kernel", "kernel_ms_per_step": round(kern_s * 1e3, 3),
+            "launches_per_step": st["n_launch"], "alg_bytes_per_step": int(alg_bytes)}
+    if args.traffic and os.path.exists(args.traffic):
+        with open(args.traffic) as f:
+            tr = json.load(f)
+        roof["traffic"] = tr.get("bytes_per_step")
+    lat = st["t_end_ns"] / 1e6
+    svc = (st["t_end_ns"] - st["t_start_ns"]) / 1e6
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "bases/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u8+int32", "data": "synthetic",
+        "config": {"workload": cfg["name"], "n_reads": len(reads), "bases": bases, "kernel": args.kernel,
+                   "graph_vertices": graph.n_vtx, "graph_seed": cfg["graph"]["seed"],
+                   "read_seed": cfg["reads"]["seed"], "parallelism": "reads across workgroups, 1 GPU"},
+        "reads_per_s": round(len(reads) * args.steps / elapsed, 1),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
+                       "service_p50": round(float(np.percentile(svc, 50)), 3),
+                       "service_p99": round(float(np.percentile(svc, 99)), 3),
+                       "definition": "batch mode: p = read's result time since the run's first read started; "
+                                     "service = result time - start time of that read"},
+        "roofline": roof,
+        "gpu_counters": {"states": st["sum_states"], "chars": st["sum_chars"], "steps": st["sum_steps"]},
+    }
+    if not args.no_cpu:
+        cb, par, ost = cpu_baseline(gw, graph, reads, args.cpu_sample, eds)
+        out["cpu_baseline"] = cb
+        out["parity_sample"] = par
+        n = par["reads"]
+        out["parity_sample"]["oracle_states"] = ost["states"]
+        if par["mismatches"]:
+            print(json.dumps(out), flush=True)
+            raise SystemExit(f"bench.py: {par['mismatches']} EDs differ from the oracle")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -126,6 +126,8 @@ def hip_lib():
         lib.gwf_dreads_free.restype = None
         lib.gwf_ed_run.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_uint32, C.c_int32,
                                    C.c_void_p, P(RunStats)]
+        lib.gwf_ed_run_timed.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_uint32, C.c_int32,
+                                         C.c_void_p, P(RunStats), C.c_void_p, C.c_void_p]
         lib.gwf_ed_batch.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                      C.c_int32, C.c_uint32, C.c_int32, C.c_void_p, P(RunStats)]
         lib.gwf_last_error.restype = C.c_char_p
@@ -404,13 +406,23 @@ class DeviceReads:
             pass
 
 
-def ed_run(dgraph, dreads, kernel=KERNEL_PERSISTENT, v1=-1):
-    """EDs of every read of dreads (np.int32) and the run statistics."""
-    out = np.empty(max(dreads.n, 1), dtype=np.int32)
+def ed_run(dgraph, dreads, kernel=KERNEL_PLAIN, v1=-1, timed=False):
+    """EDs of every read of dreads (np.int32) and the run statistics; with
+    timed=True the stats also carry per-read device start/end times (ns)."""
+    n = max(dreads.n, 1)
+    out = np.empty(n, dtype=np.int32)
     st = RunStats()
-    rc = hip_lib().gwf_ed_run(dgraph.ptr, dreads.ptr, v1, 0, kernel, out.ctypes.data, C.byref(st))
+    if timed:
+        t0, t1 = np.zeros(n, np.int64), np.zeros(n, np.int64)
+        rc = hip_lib().gwf_ed_run_timed(dgraph.ptr, dreads.ptr, v1, 0, kernel, out.ctypes.data, C.byref(st),
+                                        t0.ctypes.data, t1.ctypes.data)
+    else:
+        rc = hip_lib().gwf_ed_run(dgraph.ptr, dreads.ptr, v1, 0, kernel, out.ctypes.data, C.byref(st))
     _check(rc, "gwf_ed_run")
-    return out[:dreads.n], st.as_dict()
+    d = st.as_dict()
+    if timed:
+        d["t_start_ns"], d["t_end_ns"] = t0[:dreads.n], t1[:dreads.n]
+    return out[:dreads.n], d
 
 
 def gwf_ed(graph, q, v0=0, v1=-1):

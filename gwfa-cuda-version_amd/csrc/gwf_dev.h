@@ -49,6 +49,10 @@ struct Caps {
 	i32 h;   // forbidden-interval history (each of two buffers)
 };
 
+// Per-read counters written by the kernels: score steps, wavefront states
+// processed, characters compared, device wall-clock ticks at start and end.
+#define GWF_NCOUNT 5
+
 // Per-read result status.
 #define GWF_ST_OK        0
 #define GWF_ST_OVERFLOW  1   // a workspace capacity was exceeded; re-run larger

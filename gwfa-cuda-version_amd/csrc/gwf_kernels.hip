@@ -559,6 +559,7 @@ __global__ __launch_bounds__(WAVE) void gwf_plain_kernel(DGraph g, DReads rd, co
 	if (b >= n) return;
 	const i32 r = read_idx ? read_idx[b] : b;
 	Ws w = carve(scratch + (size_t)b * ws_elems_per_read, caps);
+	const u64 t0 = wall_clock64();
 	int64_t steps = 0, states = 0, chars = 0;
 	i32 st;
 	const i32 ed = align_read(g, rd.seq + rd.off[r], rd.ql[r], (u32)rd.v0[r], caps, w, lds_sort, vis, &st,
@@ -566,11 +567,14 @@ __global__ __launch_bounds__(WAVE) void gwf_plain_kernel(DGraph g, DReads rd, co
 	// chars is per lane; reduce
 	for (int o = 32; o > 0; o >>= 1) chars += __shfl_xor(chars, o);
 	if (lane_id() == 0) {
+		int64_t *c = out_counts + GWF_NCOUNT * (size_t)r;
 		out_ed[r] = ed;
 		out_status[r] = st;
-		out_counts[3 * (size_t)r] = steps;
-		out_counts[3 * (size_t)r + 1] = states;
-		out_counts[3 * (size_t)r + 2] = chars;
+		c[0] = steps;
+		c[1] = states;
+		c[2] = chars;
+		c[3] = (int64_t)t0;
+		c[4] = (int64_t)wall_clock64();
 	}
 }
 

@@ -64,6 +64,61 @@ struct gwf_dreads_s {
 
 namespace {
 
+// Run context: stream, events and device buffers reused across gwf_ed_run
+// calls (grown on demand), so steady-state runs allocate nothing.  Runs are
+// serialised by mu.
+struct RunCtx {
+	std::mutex mu;
+	int dev = -1;
+	int wall_khz = 0;
+	hipStream_t s = nullptr;
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	i32 *ed = nullptr, *status = nullptr, *idx = nullptr;
+	int64_t *cnt = nullptr;
+	size_t n_cap = 0;
+	u64 *scratch = nullptr;
+	size_t scratch_bytes = 0;
+};
+RunCtx g_run;
+
+template <class T>
+hipError_t grow(T **p, size_t *cap, size_t need) {
+	if (*p && *cap >= need) return hipSuccess;
+	if (*p) (void)hipFree(*p);
+	*p = nullptr;
+	*cap = 0;
+	hipError_t e = hipMalloc((void **)p, need);
+	if (e == hipSuccess) *cap = need;
+	return e;
+}
+
+int ctx_prepare(RunCtx &c, size_t n) {
+	int dev = 0;
+	HIPCHK(hipGetDevice(&dev));
+	if (c.dev != dev) {  // first use (device buffers of another device are dropped)
+		c.ed = c.status = c.idx = nullptr;
+		c.cnt = nullptr;
+		c.scratch = nullptr;
+		c.n_cap = c.scratch_bytes = 0;
+		HIPCHK(hipStreamCreate(&c.s));
+		HIPCHK(hipEventCreate(&c.e0));
+		HIPCHK(hipEventCreate(&c.e1));
+		HIPCHK(hipDeviceGetAttribute(&c.wall_khz, hipDeviceAttributeWallClockRate, dev));
+		c.dev = dev;
+	}
+	if (c.n_cap < n) {
+		const size_t m = std::max<size_t>(n, 1024);
+		size_t c0 = c.n_cap * 4, c1 = c.n_cap * 4, c2 = c.n_cap * 4, c3 = c.n_cap * 8 * GWF_NCOUNT;
+		c.n_cap = 0;
+		HIPCHK(grow(&c.ed, &c0, 4 * m));
+		HIPCHK(grow(&c.status, &c1, 4 * m));
+		HIPCHK(grow(&c.idx, &c2, 4 * m));
+		HIPCHK(grow(&c.cnt, &c3, 8 * GWF_NCOUNT * m));
+		c.n_cap = m;
+	}
+	return GWF_OK;
+}
+
 std::mutex g_cache_mu;
 std::unordered_map<const gwf_graph_t *, gwf_dgraph_t *> g_cache;
 
@@ -272,6 +327,11 @@ void gwf_dreads_free(gwf_dreads_t *dr) {
 
 int gwf_ed_run(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t v1, uint32_t max_lag, int32_t kernel, int32_t *out,
                gwf_run_stats_t *stats) {
+	return gwf_ed_run_timed(dg, dr, v1, max_lag, kernel, out, stats, nullptr, nullptr);
+}
+
+int gwf_ed_run_timed(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t v1, uint32_t max_lag, int32_t kernel, int32_t *out,
+                     gwf_run_stats_t *stats, int64_t *t_start_ns, int64_t *t_end_ns) {
 	(void)max_lag;  // accepted and ignored, as in the reference (gwf-ed.cu:1286-1287)
 	const auto t0 = std::chrono::steady_clock::now();
 	if (!dg || !dr || (!out && dr->r.n > 0)) { set_err("gwf_ed_run: NULL argument"); return GWF_ERR_ARG; }
@@ -284,16 +344,14 @@ int gwf_ed_run(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t v1, uint32_t max_lag,
 	memset(&st, 0, sizeof(st));
 	if (n == 0) { if (stats) *stats = st; return GWF_OK; }
 
-	i32 *d_ed = nullptr, *d_status = nullptr, *d_idx = nullptr;
-	int64_t *d_cnt = nullptr;
-	u64 *scratch = nullptr;
-	hipStream_t s = nullptr;
-	hipEvent_t e0 = nullptr, e1 = nullptr;
-	int rc = GWF_OK;
-	std::vector<i32> status(n), idx;
-	std::vector<int64_t> cnt(3 * (size_t)n);
-	size_t free_b = 0, tot_b = 0;
-	size_t budget, scratch_bytes = 0;
+	RunCtx &cx = g_run;
+	std::lock_guard<std::mutex> lock(cx.mu);
+	int rc = ctx_prepare(cx, (size_t)n);
+	if (rc) return rc;
+	hipStream_t s = cx.s;
+	std::vector<i32> status(n);
+	std::vector<int64_t> cnt(GWF_NCOUNT * (size_t)n);
+	size_t free_b = 0, tot_b = 0, budget = 0;
 	std::vector<i32> todo(n);
 	for (i32 i = 0; i < n; ++i) todo[i] = i;
 	int scale = 1;
@@ -308,15 +366,8 @@ int gwf_ed_run(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t v1, uint32_t max_lag,
 		}                                                                          \
 	} while (0)
 
-	RUNCHK(hipStreamCreate(&s));
-	RUNCHK(hipEventCreate(&e0));
-	RUNCHK(hipEventCreate(&e1));
-	RUNCHK(hipMalloc(&d_ed, 4 * (size_t)n));
-	RUNCHK(hipMalloc(&d_status, 4 * (size_t)n));
-	RUNCHK(hipMalloc(&d_idx, 4 * (size_t)n));
-	RUNCHK(hipMalloc(&d_cnt, 24 * (size_t)n));
 	RUNCHK(hipMemGetInfo(&free_b, &tot_b));
-	budget = std::min<size_t>(free_b / 2, (size_t)32 << 30);
+	budget = std::min<size_t>((free_b + cx.scratch_bytes) / 2, (size_t)32 << 30);
 
 	while (!todo.empty()) {
 		// batches of consecutive pending reads whose workspaces fit the budget
@@ -332,24 +383,19 @@ int gwf_ed_run(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t v1, uint32_t max_lag,
 			}
 			const Caps caps = caps_for(maxql, scale);
 			const i32 nb = (i32)(end - pos);
-			if (ws * nb > scratch_bytes) {
-				if (scratch) (void)hipFree(scratch);
-				scratch = nullptr;
-				scratch_bytes = ws * nb;
-				RUNCHK(hipMalloc(&scratch, scratch_bytes));
-			}
-			RUNCHK(hipMemcpyAsync(d_idx, todo.data() + pos, 4 * (size_t)nb, hipMemcpyHostToDevice, s));
-			RUNCHK(hipEventRecord(e0, s));
-			RUNCHK(gwf_launch_plain(dg->g, dr->r, d_idx, nb, caps, scratch, ws, d_ed, d_status, d_cnt, s));
-			RUNCHK(hipEventRecord(e1, s));
-			RUNCHK(hipEventSynchronize(e1));
+			RUNCHK(grow(&cx.scratch, &cx.scratch_bytes, ws * nb));
+			RUNCHK(hipMemcpyAsync(cx.idx, todo.data() + pos, 4 * (size_t)nb, hipMemcpyHostToDevice, s));
+			RUNCHK(hipEventRecord(cx.e0, s));
+			RUNCHK(gwf_launch_plain(dg->g, dr->r, cx.idx, nb, caps, cx.scratch, ws, cx.ed, cx.status, cx.cnt, s));
+			RUNCHK(hipEventRecord(cx.e1, s));
+			RUNCHK(hipEventSynchronize(cx.e1));
 			float ms = 0;
-			RUNCHK(hipEventElapsedTime(&ms, e0, e1));
+			RUNCHK(hipEventElapsedTime(&ms, cx.e0, cx.e1));
 			st.kernel_ms += ms;
 			st.n_launch += 1;
 			pos = end;
 		}
-		RUNCHK(hipMemcpy(status.data(), d_status, 4 * (size_t)n, hipMemcpyDeviceToHost));
+		RUNCHK(hipMemcpy(status.data(), cx.status, 4 * (size_t)n, hipMemcpyDeviceToHost));
 		std::vector<i32> again;
 		for (i32 r : todo) {
 			if (status[r] == GWF_ST_OVERFLOW) again.push_back(r);
@@ -370,23 +416,26 @@ int gwf_ed_run(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t v1, uint32_t max_lag,
 		}
 		todo.swap(again);
 	}
-	RUNCHK(hipMemcpy(out, d_ed, 4 * (size_t)n, hipMemcpyDeviceToHost));
-	RUNCHK(hipMemcpy(cnt.data(), d_cnt, 24 * (size_t)n, hipMemcpyDeviceToHost));
-	for (i32 i = 0; i < n; ++i) {
-		st.sum_steps += cnt[3 * (size_t)i];
-		st.sum_states += cnt[3 * (size_t)i + 1];
-		st.sum_chars += cnt[3 * (size_t)i + 2];
+	RUNCHK(hipMemcpy(out, cx.ed, 4 * (size_t)n, hipMemcpyDeviceToHost));
+	RUNCHK(hipMemcpy(cnt.data(), cx.cnt, 8 * GWF_NCOUNT * (size_t)n, hipMemcpyDeviceToHost));
+	{
+		int64_t tmin = INT64_MAX;
+		for (i32 i = 0; i < n; ++i) {
+			const int64_t *c = &cnt[GWF_NCOUNT * (size_t)i];
+			st.sum_steps += c[0];
+			st.sum_states += c[1];
+			st.sum_chars += c[2];
+			tmin = std::min(tmin, c[3]);
+		}
+		const double ns_per_tick = cx.wall_khz > 0 ? 1e6 / cx.wall_khz : 10.0;
+		for (i32 i = 0; i < n; ++i) {
+			const int64_t *c = &cnt[GWF_NCOUNT * (size_t)i];
+			if (t_start_ns) t_start_ns[i] = (int64_t)((c[3] - tmin) * ns_per_tick);
+			if (t_end_ns) t_end_ns[i] = (int64_t)((c[4] - tmin) * ns_per_tick);
+		}
 	}
 done:
 #undef RUNCHK
-	if (scratch) (void)hipFree(scratch);
-	if (d_ed) (void)hipFree(d_ed);
-	if (d_status) (void)hipFree(d_status);
-	if (d_idx) (void)hipFree(d_idx);
-	if (d_cnt) (void)hipFree(d_cnt);
-	if (e0) (void)hipEventDestroy(e0);
-	if (e1) (void)hipEventDestroy(e1);
-	if (s) (void)hipStreamDestroy(s);
 	st.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 	if (stats) *stats = st;
 	return rc;

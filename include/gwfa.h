@@ -175,6 +175,14 @@ void gwf_dreads_free(gwf_dreads_t *dr);
 int gwf_ed_run(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t v1, uint32_t max_lag,
                int32_t kernel, int32_t *out, gwf_run_stats_t *stats);
 
+/* gwf_ed_run plus per-read device timestamps: t_start_ns[i] and t_end_ns[i]
+ * (either may be NULL) receive the time read i started and wrote its ED, in
+ * ns since the first read of the run started (device wall clock).  A read
+ * re-run after a workspace overflow reports its final run. */
+int gwf_ed_run_timed(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t v1, uint32_t max_lag,
+                     int32_t kernel, int32_t *out, gwf_run_stats_t *stats,
+                     int64_t *t_start_ns, int64_t *t_end_ns);
+
 /* Convenience: upload reads from host pointers, run, free.  q[i] points at
  * read i (ql[i] bytes); v0 may be NULL. */
 int gwf_ed_batch(gwf_dgraph_t *dg, int32_t n, const char *const *q, const int32_t *ql,
