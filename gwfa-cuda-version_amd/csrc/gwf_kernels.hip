@@ -1,30 +1,35 @@
 // gwf_kernels.hip — CDNA4 (gfx950) kernels for GWFA graph edit distance.
 //
-// One 64-lane wavefront aligns one read.  Per score step s (gwf_ed_extend,
-// gwf-ed.cu:1064-1290) the wave:
-//   1. extends every wavefront state (gwf_extend1, gwf-ed.cu:806-839): each
+// A workgroup of NW 64-lane wavefronts aligns one read.  Per score step s
+// (gwf_ed_extend, gwf-ed.cu:1064-1290) it:
+//   P1 extends every wavefront state (gwf_extend1, gwf-ed.cu:806-839): each
 //      lane compares 16 bytes with two 8-byte XORs; states still matching are
-//      then extended cooperatively, 512 bytes per wave step (8 bytes per lane,
-//      a 64-bit ballot and __ffsll pick the first mismatching lane);
-//   2. emits the in-vertex successors (the WFA recurrence of
+//      then extended cooperatively by their wave, 512 bytes per wave step
+//      (8 bytes per lane; a 64-bit ballot and __ffsll pick the first
+//      mismatching lane);
+//   P2 emits the in-vertex successors (the WFA recurrence of
 //      gwf_ed_extend_batch, gwf-ed.cu:891-1018) directly in sorted order: the
 //      state on diagonal d owns output diagonals d-1 / d / d+1 unless a
-//      neighbour owns them, and takes the max over its neighbours' candidates;
-//   3. runs the vertex-crossing closure (the A queue, gwf-ed.cu:1106-1255)
-//      level by level over a frontier, reading the CSR arcs of each vertex;
-//   4. sorts the closure's successors (bitonic in LDS), merges them with the
-//      in-vertex successors, keeps the furthest state per diagonal
-//      (gwf_diag_dedup, gwf-ed.cu:653-687) and drops diagonals inside the
-//      forbidden-interval history (gwf_dedup / gwf_mixed_dedup,
-//      gwf-ed.cu:691-804), which is updated with this step's exits.
-// All arithmetic is exact integer; every buffer write is bounds-checked and a
-// read that outgrows its workspace reports GWF_ST_OVERFLOW (the host re-runs
-// it with a larger one) instead of dropping state.
+//      neighbour owns them and takes the max over its neighbours' candidates;
+//   P3 runs the vertex-crossing closure (the A queue, gwf-ed.cu:1106-1255)
+//      level by level over a frontier, reading each vertex's CSR arcs;
+//   P4 sorts the closure's successors (bitonic) and P6 merges them in place
+//      into the in-vertex successors (merge path, right to left);
+//   P5 folds this step's exits into the forbidden-interval history
+//      (gwf_dedup, gwf-ed.cu:780-804);
+//   P7 keeps the furthest state of every diagonal (gwf_diag_dedup,
+//      gwf-ed.cu:653-687) outside the forbidden intervals (gwf_mixed_dedup,
+//      gwf-ed.cu:691-747).
+// The same step code runs on arrays in LDS (persistent kernel) or in HBM
+// (plain kernel, and a persistent read whose wavefront outgrew its LDS).  A
+// step that would overflow LDS leaves its input untouched and is re-run in
+// HBM; an HBM overflow reports GWF_ST_OVERFLOW and the host re-runs the read
+// with a larger workspace.  All arithmetic is exact integer.
 #include "gwf_dev.h"
 
 #define WAVE 64
-#define VIS_SLOTS 512     // per-step visited set (w, i+1), LDS
-#define SORT_LDS 1024     // largest run sorted in LDS in one piece
+#define VIS_SLOTS 256      // per-step visited set (w, i+1), LDS
+#define SORT_CHUNK 1024    // plain kernel: run length sorted in LDS
 #define EMPTY_KEY (~0ull)
 
 // ------------------------------------------------------------ key helpers
@@ -44,15 +49,13 @@ __device__ __forceinline__ u32 rank_below(u64 mask) {
 	return __builtin_amdgcn_mbcnt_hi((u32)(mask >> 32), __builtin_amdgcn_mbcnt_lo((u32)mask, 0u));
 }
 
-__device__ __forceinline__ i32 uni(i32 x) { return __builtin_amdgcn_readfirstlane(x); }
-
 // exclusive prefix sum over the wave; *total receives the wave sum
-__device__ __forceinline__ i32 scan_excl(i32 x, i32 *total) {
+__device__ __forceinline__ i32 wave_scan(i32 x, i32 *total) {
 	const u32 lane = lane_id();
 	i32 incl = x;
 #pragma unroll
 	for (int o = 1; o < WAVE; o <<= 1) {
-		i32 y = __shfl_up(incl, o);
+		const i32 y = __shfl_up(incl, o);
 		if (lane >= (u32)o) incl += y;
 	}
 	*total = __shfl(incl, WAVE - 1);
@@ -60,23 +63,118 @@ __device__ __forceinline__ i32 scan_excl(i32 x, i32 *total) {
 }
 
 // inclusive prefix max over the wave (u64)
-__device__ __forceinline__ u64 scan_max_incl(u64 x) {
+__device__ __forceinline__ u64 wave_scan_max(u64 x) {
 	const u32 lane = lane_id();
 #pragma unroll
 	for (int o = 1; o < WAVE; o <<= 1) {
-		u64 y = __shfl_up(x, o);
+		const u64 y = __shfl_up(x, o);
 		if (lane >= (u32)o) x = x > y ? x : y;
 	}
 	return x;
 }
 
-__device__ __forceinline__ void wsync() { __syncthreads(); }
+// ------------------------------------------------------------ block helpers
+// Collectives over the NW waves of a workgroup.  Every thread must call them
+// the same number of times (trip counts of loops around them are uniform).
+template <int NW>
+struct Blk {
+	static constexpr int NT = WAVE * NW;
+	i32 *red;   // LDS scratch, NW ints
+	u64 *redu;  // LDS scratch, NW u64
+	__device__ __forceinline__ u32 tid() const { return threadIdx.x; }
+	__device__ __forceinline__ u32 wid() const { return NW == 1 ? 0u : threadIdx.x / WAVE; }
+	__device__ __forceinline__ void sync() const { __syncthreads(); }
+	// prefix over waves of a per-wave value (uniform within each wave)
+	__device__ __forceinline__ i32 wave_prefix(i32 c, i32 *tot) const {
+		if constexpr (NW == 1) {
+			*tot = c;
+			return 0;
+		} else {
+			if (lane_id() == 0) red[wid()] = c;
+			sync();
+			i32 pre = 0, t = 0;
+#pragma unroll
+			for (int w = 0; w < NW; ++w) {
+				const i32 v = red[w];
+				pre += w < (int)wid() ? v : 0;
+				t += v;
+			}
+			sync();
+			*tot = t;
+			return pre;
+		}
+	}
+	__device__ __forceinline__ i32 scan(i32 x, i32 *tot) const {
+		i32 wt, t;
+		const i32 p = wave_scan(x, &wt);
+		const i32 pre = wave_prefix(wt, &t);
+		*tot = t;
+		return p + pre;
+	}
+	// compaction slot of a predicate; *n advances by the block count
+	__device__ __forceinline__ i32 slot(bool pred, i32 *n) const {
+		const u64 m = __ballot(pred);
+		i32 t;
+		const i32 pre = wave_prefix(__popcll(m), &t);
+		const i32 s = *n + pre + (i32)rank_below(m);
+		*n += t;
+		return s;
+	}
+	__device__ __forceinline__ bool any(bool p) const {
+		if constexpr (NW == 1) return __ballot(p) != 0;
+		else return __syncthreads_or(p) != 0;
+	}
+	__device__ __forceinline__ i32 max(i32 x) const {
+#pragma unroll
+		for (int o = 32; o > 0; o >>= 1) {
+			const i32 y = __shfl_xor(x, o);
+			x = x > y ? x : y;
+		}
+		if constexpr (NW == 1) {
+			return x;
+		} else {
+			if (lane_id() == 0) red[wid()] = x;
+			sync();
+			i32 m = red[0];
+#pragma unroll
+			for (int w = 1; w < NW; ++w) m = m > red[w] ? m : red[w];
+			sync();
+			return m;
+		}
+	}
+	// inclusive prefix max (u64) over the block; *excl = exclusive prefix max
+	// (0 for the first thread), *tot = block max
+	__device__ __forceinline__ u64 scan_max2(u64 x, u64 *excl, u64 *tot) const {
+		const u64 wi = wave_scan_max(x);
+		u64 we = __shfl_up(wi, 1);
+		if (lane_id() == 0) we = 0;
+		if constexpr (NW == 1) {
+			*excl = we;
+			*tot = __shfl(wi, WAVE - 1);
+			return wi;
+		} else {
+			if (lane_id() == WAVE - 1) redu[wid()] = wi;
+			sync();
+			u64 pre = 0, t = 0;
+#pragma unroll
+			for (int w = 0; w < NW; ++w) {
+				const u64 v = redu[w];
+				if (w < (int)wid()) pre = pre > v ? pre : v;
+				t = t > v ? t : v;
+			}
+			sync();
+			*excl = we > pre ? we : pre;
+			*tot = t;
+			return wi > pre ? wi : pre;
+		}
+	}
+};
 
 // unaligned 8-byte load from a buffer padded past its end
 __device__ __forceinline__ u64 load8u(const uint8_t *p) {
-	const uintptr_t a = (uintptr_t)p;
-	const u64 *w = (const u64 *)(a & ~(uintptr_t)7);
-	const u32 sh = (u32)(a & 7) * 8;
+	const u32 mis = (u32)((uintptr_t)p & 7);
+	const u64 *w = (const u64 *)(p - mis);  // pointer arithmetic keeps the address space
+	const u32 sh = mis * 8;
 	const u64 lo = w[0], hi = w[1];
 	return sh ? (lo >> sh) | (hi << (64 - sh)) : lo;
 }
@@ -85,27 +183,34 @@ __device__ __forceinline__ u64 load8u(const uint8_t *p) {
 __device__ __forceinline__ i32 lower_bound(const u64 *a, i32 n, u64 x) {
 	i32 lo = 0, hi = n;
 	while (lo < hi) {
-		i32 m = (lo + hi) >> 1;
-		if (a[m] < x) lo = m + 1; else hi = m;
+		const i32 m = (lo + hi) >> 1;
+		if (a[m] < x) lo = m + 1;
+		else hi = m;
 	}
 	return lo;
 }
 __device__ __forceinline__ i32 upper_bound(const u64 *a, i32 n, u64 x) {
 	i32 lo = 0, hi = n;
 	while (lo < hi) {
-		i32 m = (lo + hi) >> 1;
-		if (a[m] <= x) lo = m + 1; else hi = m;
+		const i32 m = (lo + hi) >> 1;
+		if (a[m] <= x) lo = m + 1;
+		else hi = m;
 	}
 	return lo;
 }
 
+__device__ __forceinline__ i32 next_pow2(i32 x) {
+	i32 n = 2;
+	while (n < x) n <<= 1;
+	return n;
+}
+
 // ------------------------------------------------------------ extension
-// LCE of graph bytes ts[k+1..] and query bytes qd[k+1..] (qd = q + d) up to
-// max_k: the signed-max_k form of gwf_extend1 (gwf-ed.cu:806-839).  Every
-// lane passes its own state (act = false lanes pass through); all lanes must
-// call it together.
-__device__ i32 extend_wave(bool act, const uint8_t *seq, u32 seq_off, const uint8_t *q, i32 d, i32 k,
-                           i32 max_k, int64_t *chars) {
+// LCE of graph bytes ts[k+1..] and query bytes q[d+k+1..] up to max_k: the
+// signed-max_k form of gwf_extend1 (gwf-ed.cu:806-839).  All lanes of the
+// wave call it together; lanes with act == false pass through.
+__device__ __forceinline__ i32 extend_wave(bool act, const uint8_t *seq, u32 seq_off, const uint8_t *q, i32 d, i32 k, i32 max_k,
+                           int64_t *chars) {
 	bool more = false;
 	if (act) {
 		const uint8_t *ts = seq + seq_off;
@@ -132,11 +237,10 @@ __device__ i32 extend_wave(bool act, const uint8_t *seq, u32 seq_off, const uint
 		const u32 so = (u32)__shfl((i32)seq_off, src);
 		const uint8_t *ts = seq + so;
 		const uint8_t *qd = q + dd;
-		i32 base = kk;
-		i32 res;
+		i32 base = kk, res;
 		for (;;) {
-			const i32 avail = mk_ - base;              // chars still allowed
-			const i32 my = (i32)lane * 8;              // lane's first char (relative)
+			const i32 avail = mk_ - base;
+			const i32 my = (i32)lane * 8;
 			i32 m = 8;
 			if (my < avail) {
 				const u64 x = load8u(ts + base + 1 + my) ^ load8u(qd + base + 1 + my);
@@ -153,120 +257,189 @@ __device__ i32 extend_wave(bool act, const uint8_t *seq, u32 seq_off, const uint
 			if (avail <= 0) { res = base; break; }
 			base += WAVE * 8;
 		}
-		if ((int)lane == src) { *chars += res - kk + (res < mk_ ? 1 : 0); k = res; }
+		if ((int)lane == src) {
+			*chars += res - kk + (res < mk_ ? 1 : 0);
+			k = res;
+		}
 	}
 	return k;
 }
 
-// ------------------------------------------------------------ sorting/merging
-// Bitonic sort of lds[0..N) (N a power of two, padded with EMPTY_KEY).
-__device__ void bitonic_lds(u64 *lds, i32 N) {
-	const u32 lane = lane_id();
+// ------------------------------------------------------------ sort / merge
+// Bitonic sort of a[0..N), N a power of two.
+template <int NW>
+__device__ __forceinline__ void bitonic(u64 *a, i32 N, const Blk<NW> &b) {
 	for (i32 size = 2; size <= N; size <<= 1) {
 		for (i32 stride = size >> 1; stride > 0; stride >>= 1) {
-			for (i32 t = lane; t < (N >> 1); t += WAVE) {
+			for (i32 t = b.tid(); t < (N >> 1); t += Blk<NW>::NT) {
 				const i32 i = 2 * t - (t & (stride - 1));
 				const i32 j = i + stride;
 				const bool up = (i & size) == 0;
-				const u64 a = lds[i], b = lds[j];
-				if ((a > b) == up) { lds[i] = b; lds[j] = a; }
+				const u64 x = a[i], y = a[j];
+				if ((x > y) == up) { a[i] = y; a[j] = x; }
 			}
-			wsync();
+			b.sync();
 		}
 	}
 }
 
 // Merge sorted a[0..na) and b[0..nb) into out (a before b on ties).
-__device__ void merge_keys(const u64 *a, i32 na, const u64 *b, i32 nb, u64 *out) {
-	const u32 lane = lane_id();
-	for (i32 j = lane; j < na; j += WAVE) {
+template <int NW>
+__device__ __forceinline__ void merge_keys(const u64 *a, i32 na, const u64 *c, i32 nc, u64 *out, const Blk<NW> &b) {
+	for (i32 j = b.tid(); j < na; j += Blk<NW>::NT) {
 		const u64 x = a[j];
-		out[j + lower_bound(b, nb, x)] = x;
+		out[j + lower_bound(c, nc, x)] = x;
 	}
-	for (i32 t = lane; t < nb; t += WAVE) {
-		const u64 x = b[t];
+	for (i32 t = b.tid(); t < nc; t += Blk<NW>::NT) {
+		const u64 x = c[t];
 		out[t + upper_bound(a, na, x)] = x;
 	}
-	wsync();
+	b.sync();
 }
 
-// Sort p[0..n) ascending; lds holds SORT_LDS keys; tmp is a global buffer of n keys.
-__device__ void sort_keys(u64 *p, i32 n, u64 *lds, u64 *tmp) {
-	const u32 lane = lane_id();
+// Sort p[0..n) (HBM): runs of `lc` sorted in the LDS scratch, then pairwise
+// merges through tmp (n elements).
+template <int NW>
+__device__ __forceinline__ void sort_hbm(u64 *p, i32 n, u64 *lds, i32 lc, u64 *tmp, const Blk<NW> &b) {
 	if (n <= 1) return;
-	for (i32 c0 = 0; c0 < n; c0 += SORT_LDS) {  // sort runs of SORT_LDS in LDS
-		const i32 m = n - c0 < SORT_LDS ? n - c0 : SORT_LDS;
-		i32 N = 2;
-		while (N < m) N <<= 1;
-		for (i32 t = lane; t < N; t += WAVE) lds[t] = t < m ? p[c0 + t] : EMPTY_KEY;
-		wsync();
-		bitonic_lds(lds, N);
-		for (i32 t = lane; t < m; t += WAVE) p[c0 + t] = lds[t];
-		wsync();
+	for (i32 c0 = 0; c0 < n; c0 += lc) {
+		const i32 m = n - c0 < lc ? n - c0 : lc;
+		const i32 N = next_pow2(m);
+		for (i32 t = b.tid(); t < N; t += Blk<NW>::NT) lds[t] = t < m ? p[c0 + t] : EMPTY_KEY;
+		b.sync();
+		bitonic(lds, N, b);
+		for (i32 t = b.tid(); t < m; t += Blk<NW>::NT) p[c0 + t] = lds[t];
+		b.sync();
 	}
 	u64 *src = p, *dst = tmp;
-	for (i32 w = SORT_LDS; w < n; w <<= 1) {  // pairwise merges in global memory
+	for (i32 w = lc; w < n; w <<= 1) {
 		for (i32 l = 0; l < n; l += 2 * w) {
 			const i32 mid = l + w < n ? l + w : n;
 			const i32 r = l + 2 * w < n ? l + 2 * w : n;
-			merge_keys(src + l, mid - l, src + mid, r - mid, dst + l);
+			merge_keys(src + l, mid - l, src + mid, r - mid, dst + l, b);
 		}
-		u64 *t = src; src = dst; dst = t;
+		u64 *t = src;
+		src = dst;
+		dst = t;
 	}
 	if (src != p) {
-		for (i32 t = lane; t < n; t += WAVE) p[t] = src[t];
-		wsync();
+		for (i32 t = b.tid(); t < n; t += Blk<NW>::NT) p[t] = src[t];
+		b.sync();
 	}
 }
 
-// Merge interval lists (alo/ahi by alo) and unit intervals b (as [b, b+1))
-// into olo/ohi.
-__device__ void merge_intervals(const u64 *alo, const u64 *ahi, i32 na, const u64 *b, i32 nb, u64 *olo,
-                                u64 *ohi) {
+// Bitonic sort of 64 keys held one per lane (ascending by lane).
+__device__ __forceinline__ u64 wave_sort64(u64 x) {
 	const u32 lane = lane_id();
-	for (i32 j = lane; j < na; j += WAVE) {
+#pragma unroll
+	for (int size = 2; size <= WAVE; size <<= 1) {
+#pragma unroll
+		for (int stride = size >> 1; stride > 0; stride >>= 1) {
+			const u64 y = __shfl_xor(x, stride);
+			const bool up = (lane & size) == 0;
+			const bool lower = (lane & stride) == 0;
+			const u64 lo = x < y ? x : y, hi = x < y ? y : x;
+			x = (lower == up) ? lo : hi;
+		}
+	}
+	return x;
+}
+
+// Sort src[0..n) into dst: every wave sorts runs of 64 in registers (no
+// barriers), then each key's final rank is found by binary search in the
+// other runs (up to 8 runs), or runs are merged pairwise through dst/src.
+template <int NW>
+__device__ __forceinline__ void sort_runs(u64 *src, i32 n, u64 *dst, const Blk<NW> &b) {
+	const i32 k = (n + WAVE - 1) / WAVE;
+	for (i32 c = (i32)b.wid(); c < k; c += NW) {
+		const i32 j = c * WAVE + (i32)lane_id();
+		u64 x = j < n ? src[j] : EMPTY_KEY;
+		x = wave_sort64(x);
+		if (j < n) src[j] = x;
+	}
+	b.sync();
+	if (k <= 1) {
+		for (i32 j = (i32)b.tid(); j < n; j += Blk<NW>::NT) dst[j] = src[j];
+		b.sync();
+		return;
+	}
+	if (k <= 8) {
+		for (i32 j = (i32)b.tid(); j < n; j += Blk<NW>::NT) {
+			const u64 x = src[j];
+			const i32 r = j / WAVE;
+			i32 rank = j - r * WAVE;
+			for (i32 o = 0; o < k; ++o) {
+				if (o == r) continue;
+				const i32 len = (o + 1) * WAVE <= n ? WAVE : n - o * WAVE;
+				rank += o < r ? upper_bound(src + o * WAVE, len, x) : lower_bound(src + o * WAVE, len, x);
+			}
+			dst[rank] = x;
+		}
+		b.sync();
+		return;
+	}
+	u64 *a = src, *c = dst;
+	for (i32 wdt = WAVE; wdt < n; wdt <<= 1) {
+		for (i32 l = 0; l < n; l += 2 * wdt) {
+			const i32 mid = l + wdt < n ? l + wdt : n;
+			const i32 r = l + 2 * wdt < n ? l + 2 * wdt : n;
+			merge_keys(a + l, mid - l, a + mid, r - mid, c + l, b);
+		}
+		u64 *t = a;
+		a = c;
+		c = t;
+	}
+	if (a != dst) {
+		for (i32 j = (i32)b.tid(); j < n; j += Blk<NW>::NT) dst[j] = a[j];
+		b.sync();
+	}
+}
+
+// Merge two interval lists sorted by lo (a before c on ties) into olo/ohi.
+template <int NW>
+__device__ __forceinline__ void merge_intervals2(const u64 *alo, const u64 *ahi, i32 na, const u64 *clo,
+                                                 const u64 *chi, i32 nc, u64 *olo, u64 *ohi, const Blk<NW> &b) {
+	for (i32 j = (i32)b.tid(); j < na; j += Blk<NW>::NT) {
 		const u64 x = alo[j];
-		const i32 p = j + lower_bound(b, nb, x);
+		const i32 p = j + lower_bound(clo, nc, x);
 		olo[p] = x, ohi[p] = ahi[j];
 	}
-	for (i32 t = lane; t < nb; t += WAVE) {
-		const u64 x = b[t];
+	for (i32 t = (i32)b.tid(); t < nc; t += Blk<NW>::NT) {
+		const u64 x = clo[t];
 		const i32 p = t + upper_bound(alo, na, x);
-		olo[p] = x, ohi[p] = x + 1;
+		olo[p] = x, ohi[p] = chi[t];
 	}
-	wsync();
+	b.sync();
 }
 
 // Coalesce overlapping or adjacent intervals of a list sorted by lo
-// (gwf_intv_merge_adj, gwf-ed.cu:266-296).  Returns the count, or -1 if it
-// exceeds cap.
-__device__ i32 coalesce(const u64 *lo, const u64 *hi, i32 n, u64 *olo, u64 *ohi, i32 cap) {
-	const u32 lane = lane_id();
-	u64 carry = 0;      // max hi over previous chunks
-	i32 ng = 0;         // groups started so far
-	bool ovf = false;
-	for (i32 base = 0; base < n; base += WAVE) {
-		const i32 j = base + lane;
+// (gwf_intv_merge_adj, gwf-ed.cu:266-296).  Returns the output count (the
+// caller guarantees room for n).
+template <int NW>
+__device__ __forceinline__ i32 coalesce(const u64 *lo, const u64 *hi, i32 n, u64 *olo, u64 *ohi, const Blk<NW> &b) {
+	u64 carry = 0;  // max hi over previous chunks
+	i32 ng = 0;     // groups started so far
+	for (i32 base = 0; base < n; base += Blk<NW>::NT) {
+		const i32 j = base + (i32)b.tid();
 		const bool act = j < n;
 		const u64 l = act ? lo[j] : EMPTY_KEY;
 		const u64 h = act ? hi[j] : 0;
-		u64 pin = scan_max_incl(h);
+		u64 ex, tmax;
+		u64 pin = b.scan_max2(h, &ex, &tmax);
 		pin = pin > carry ? pin : carry;
-		u64 pex = __shfl_up(pin, 1);
-		if (lane == 0) pex = carry;
+		const u64 pex = ex > carry ? ex : carry;
 		const bool start = act && (j == 0 || l > pex);
 		const u64 nl = (act && j + 1 < n) ? lo[j + 1] : EMPTY_KEY;
 		const bool end = act && (j + 1 == n || nl > pin);
-		const u64 sm = __ballot(start);
-		const i32 g = ng + (i32)rank_below(sm) + (start ? 1 : 0) - 1;
-		if (g >= cap && act) ovf = true;
-		if (!ovf && start) olo[g] = l;
-		if (!ovf && end) ohi[g] = pin;
-		ng += __popcll(sm);
-		carry = __shfl(pin, WAVE - 1);
+		i32 cnt;
+		const i32 e = b.scan(start ? 1 : 0, &cnt);
+		if (start) olo[ng + e] = l;
+		if (end) ohi[ng + e + (start ? 1 : 0) - 1] = pin;
+		ng += cnt;
+		carry = carry > tmax ? carry : tmax;
 	}
-	wsync();
-	return __ballot(ovf) ? -1 : ng;
+	b.sync();
+	return ng;
 }
 
 __device__ __forceinline__ bool forbidden(const u64 *hlo, const u64 *hhi, i32 nh, u64 dg) {
@@ -286,8 +459,7 @@ __device__ __forceinline__ u32 vis_hash(u64 x) {
 __device__ __forceinline__ bool vis_insert(u64 *vis, u64 key) {
 	u32 h = vis_hash(key) & (VIS_SLOTS - 1);
 	for (int p = 0; p < 16; ++p) {
-		const u64 old = atomicCAS((unsigned long long *)&vis[h], (unsigned long long)EMPTY_KEY,
-		                          (unsigned long long)key);
+		const u64 old = atomicCAS((unsigned long long *)&vis[h], (unsigned long long)EMPTY_KEY, (unsigned long long)key);
 		if (old == EMPTY_KEY) return true;
 		if (old == key) return false;
 		h = (h + 1) & (VIS_SLOTS - 1);
@@ -295,70 +467,182 @@ __device__ __forceinline__ bool vis_insert(u64 *vis, u64 key) {
 	return true;
 }
 
-// ------------------------------------------------------------ workspace
-struct Ws {
-	u64 *A, *S, *M, *X, *F0, *F1, *T, *Tb, *Clo, *Chi, *H0lo, *H0hi, *H1lo, *H1hi, *SG;
+// ------------------------------------------------------------ working arrays
+struct Bufs {
+	u64 *P, *Q, *X, *F0, *F1, *T;
+	u64 *H0lo, *H0hi, *H1lo, *H1hi;  // forbidden-interval history, double buffered
+	u64 *sort_lds;    // HBM mode: LDS scratch for sorting runs
+	i32 sort_cap;     // its capacity (power of two)
+	u64 *sort_tmp;    // HBM mode: merge temp (max(x, t) elements)
+	Caps c;
 };
 
 __host__ __device__ inline size_t ws_elems(const Caps &c) {
-	return (size_t)c.a + c.s + (size_t)(c.s + c.x) + c.x + 2 * (size_t)c.f + 2 * (size_t)c.t +
-	       2 * (size_t)(c.h + c.t) + 4 * (size_t)c.h + (size_t)(c.x > c.t ? c.x : c.t) + 8;
+	return 2 * (size_t)c.m + c.x + 2 * (size_t)c.f + c.t + 4 * (size_t)c.h + (size_t)(c.x > c.t ? c.x : c.t) + 8;
 }
-
 size_t gwf_ws_bytes(const Caps &c) { return ((ws_elems(c) * 8 + 255) / 256) * 256; }
+__host__ __device__ inline size_t lds_elems(const Caps &c) {
+	return 2 * (size_t)c.m + c.x + 2 * (size_t)c.f + c.t + 4 * (size_t)c.h;
+}
+size_t gwf_lds_bytes(const Caps &c) { return 8 * lds_elems(c); }
 
-__device__ inline Ws carve(u64 *base, const Caps &c) {
-	Ws w;
-	u64 *p = base;
-	w.A = p; p += c.a;
-	w.S = p; p += c.s;
-	w.M = p; p += c.s + c.x;
+__device__ inline Bufs carve(u64 *p, const Caps &c, bool with_tmp) {
+	Bufs w;
+	w.c = c;
+	w.P = p; p += c.m;
+	w.Q = p; p += c.m;
 	w.X = p; p += c.x;
 	w.F0 = p; p += c.f;
 	w.F1 = p; p += c.f;
 	w.T = p; p += c.t;
-	w.Tb = p; p += c.t;
-	w.Clo = p; p += c.h + c.t;
-	w.Chi = p; p += c.h + c.t;
 	w.H0lo = p; p += c.h;
 	w.H0hi = p; p += c.h;
 	w.H1lo = p; p += c.h;
 	w.H1hi = p; p += c.h;
-	w.SG = p;
+	w.sort_tmp = with_tmp ? p : nullptr;
+	w.sort_lds = nullptr;
+	w.sort_cap = 0;
 	return w;
 }
 
-// wave-append helper: returns this lane's slot (valid if pred) and advances *n
-__device__ __forceinline__ i32 wave_slot(bool pred, i32 *n) {
-	const u64 m = __ballot(pred);
-	const i32 s = *n + (i32)rank_below(m);
-	*n += __popcll(m);
-	return s;
-}
+enum { STEP_NEXT = 0, STEP_FOUND = 1, STEP_EMPTY = 2, STEP_SPILL = 3 };
 
-// ------------------------------------------------------------ one read
-// Returns the ED (>= 0), -1 if the wavefront empties, or sets *status.
-__device__ i32 align_read(const DGraph &g, const uint8_t *q, i32 ql, u32 v0, const Caps &cap, Ws &w,
-                          u64 *lds_sort, u64 *vis, i32 *status, int64_t *n_steps, int64_t *n_states,
-                          int64_t *n_chars) {
-	const u32 lane = lane_id();
-	i32 n_a = 1, n_h = 0;
-	u64 *Hlo = w.H0lo, *Hhi = w.H0hi, *Hlo2 = w.H1lo, *Hhi2 = w.H1hi;
-	int64_t chars = 0;
-	if (lane == 0) w.A[0] = mk(v0, 0, -1);
-	wsync();
-	*status = GWF_ST_OK;
-	for (i32 s = 0;; ++s) {
-		*n_steps += 1;
-		*n_states += n_a;
-		if (s > ql + 1) { *status = GWF_ST_LIMIT; return -1; }
-		// ---- 1. extend, detect query end, collect vertex ends
-		bool found = false;
-		i32 n_f = 0;
-		for (i32 base = 0; base < n_a; base += WAVE) {
-			const i32 j = base + lane;
-			const bool act = j < n_a;
-			const u64 key = act ? w.A[j] : 0;
+// One score step.  Reads P[0..n_a) (the wavefront, sorted) and the history
+// H[hs] (n_h intervals); on STEP_NEXT leaves the next wavefront in P and the
+// updated history in H[hs ^ 1].  On STEP_SPILL nothing the next attempt needs
+// was modified (P holds the same states, extended in place, which is
+// idempotent).  HBM selects how sorts are staged.
+template <int NW, bool HBM>
+__device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 ql, const Bufs &w, const Blk<NW> &b,
+                                        u64 *vis, i32 &n_a, i32 &n_h, i32 &hs, int64_t &chars) {
+	constexpr i32 NT = Blk<NW>::NT;
+	const i32 tid = (i32)b.tid();
+	const Caps &cap = w.c;
+
+	// ---- P1: extend, detect the query end, collect vertex ends (batch frontier)
+	bool found = false;
+	i32 n_f = 0;
+	for (i32 base = 0; base < n_a; base += NT) {
+		const i32 j = base + tid;
+		const bool act = j < n_a;
+		const u64 key = act ? w.P[j] : 0;
+		const u32 v = key_v(key);
+		const i32 d = key_d(key);
+		i32 k = key_k(key);
+		const uint4 vi = act ? g.vinfo[v] : make_uint4(0, 0, 0, 0);
+		const i32 vl = (i32)vi.y;
+		const i32 max_k = (ql - d < vl ? ql - d : vl) - 1;
+		k = extend_wave(act, g.seq, vi.x, q, d, k, max_k, &chars);
+		if (act) w.P[j] = mk(v, d, k);
+		const bool qend = act && d + k == ql - 1;
+		found |= qend;
+		const bool vend = act && !qend && k == vl - 1;
+		const i32 sl = b.slot(vend, &n_f);
+		if (vend && sl < cap.f) w.F0[sl] = mk(v, d, k);
+	}
+	if (b.any(found)) return STEP_FOUND;
+	if (n_f > cap.f) return STEP_SPILL;
+
+	// ---- P2: in-vertex successors into Q (sorted) and their exits into T
+	i32 n_s = 0, n_t = 0;
+	bool ovf = false;
+	for (i32 base = 0; base < n_a; base += NT) {
+		const i32 j = base + tid;
+		const bool act = j < n_a;
+		const u64 cur = act ? w.P[j] : 0;
+		const u64 prv = (act && j > 0) ? w.P[j - 1] : EMPTY_KEY;
+		const u64 nxt = (act && j + 1 < n_a) ? w.P[j + 1] : EMPTY_KEY;
+		const u32 v = key_v(cur);
+		const i32 d = key_d(cur), k = key_k(cur);
+		const i32 vl = act ? (i32)g.vinfo[v].y : 0;
+		const bool hp = act && j > 0 && key_v(prv) == v;
+		const bool hn = act && j + 1 < n_a && key_v(nxt) == v;
+		const i32 dp = key_d(prv), kp = key_k(prv), dn = key_d(nxt), kn = key_k(nxt);
+		const bool e0 = act && !(hp && dp >= d - 2);
+		const i32 k0 = k + 1;
+		i32 k1 = k + 1;
+		if (hp && dp == d - 1 && kp > k1) k1 = kp;
+		if (hn && dn == d + 1 && kn + 1 > k1) k1 = kn + 1;
+		const bool e1 = act;
+		const bool e2 = act && !(hn && dn == d + 1);
+		i32 k2 = k;
+		if (hn && dn == d + 2 && kn + 1 > k2) k2 = kn + 1;
+		const bool s0 = e0 && (d - 1) + k0 < ql && k0 < vl, t0 = e0 && k0 == vl;
+		const bool s1 = e1 && d + k1 < ql && k1 < vl, t1 = e1 && k1 == vl;
+		const bool s2 = e2 && (d + 1) + k2 < ql && k2 < vl, t2 = e2 && k2 == vl;
+		i32 tot;
+		i32 p = n_s + b.scan((i32)s0 + (i32)s1 + (i32)s2, &tot);
+		if (n_s + tot > cap.m) ovf = true;
+		else {
+			if (s0) w.Q[p++] = mk(v, d - 1, k0);
+			if (s1) w.Q[p++] = mk(v, d, k1);
+			if (s2) w.Q[p++] = mk(v, d + 1, k2);
+		}
+		n_s += tot;
+		p = n_t + b.scan((i32)t0 + (i32)t1 + (i32)t2, &tot);
+		if (n_t + tot > cap.t) ovf = true;
+		else {
+			if (t0) w.T[p++] = diag_id(v, d - 1);
+			if (t1) w.T[p++] = diag_id(v, d);
+			if (t2) w.T[p++] = diag_id(v, d + 1);
+		}
+		n_t += tot;
+	}
+	if (ovf) return STEP_SPILL;  // (ovf is uniform: derived from block totals)
+	const i32 n_t1 = n_t;  // exits of in-vertex successors (sorted, unique)
+
+	// ---- P3: vertex-crossing closure
+	for (i32 t = tid; t < VIS_SLOTS; t += NT) vis[t] = EMPTY_KEY;
+	b.sync();
+	i32 n_x = 0;
+	bool closure_origin = false;
+	while (n_f > 0) {
+		i32 n_na = 0;
+		for (i32 base = 0; base < n_f; base += NT) {
+			const i32 j = base + tid;
+			const bool act = j < n_f;
+			const u64 key = act ? w.F0[j] : 0;
+			const u32 v = key_v(key);
+			const i32 d = key_d(key), k = key_k(key);
+			const i32 i = d + k;
+			const uint4 vi = act ? g.vinfo[v] : make_uint4(0, 0, 0, 0);
+			const bool rec = act && closure_origin;
+			const uint8_t c = act ? q[i + 1] : 0;
+			const i32 deg = act ? (i32)vi.w : 0;
+			const i32 maxdeg = b.max(deg);
+			i32 n_ext = 0;
+			for (i32 e = 0; e < maxdeg; ++e) {
+				const bool has = e < deg;
+				const uint4 ar = has ? g.arc[vi.z + e] : make_uint4(0, 0, 0, 0);
+				const u32 wv = ar.x;
+				const i32 ol = (i32)(ar.y & 0xFFFFFFu);
+				const bool m = has && (uint8_t)(ar.y >> 24) == c;
+				n_ext += m;
+				const bool absent = has && vis_insert(vis, (u64)wv << 32 | (u32)(i + 1));
+				const bool pa = absent && m;
+				const i32 sa = b.slot(pa, &n_na);
+				if (pa && sa < cap.f) w.F1[sa] = mk(wv, i + 1 - ol, ol);
+				const bool px = absent && !m;
+				i32 tx;
+				const i32 sx = n_x + b.scan(px ? 2 : 0, &tx);
+				n_x += tx;
+				if (px && sx + 1 < cap.x) {
+					w.X[sx] = mk(wv, i - ol, ol);
+					w.X[sx + 1] = mk(wv, i + 1 - ol, ol);
+				}
+			}
+			const bool pi = rec && (deg == 0 || n_ext != deg);
+			const i32 si = b.slot(pi, &n_x);
+			if (pi && si < cap.x) w.X[si] = mk(v, d + 1, k);
+		}
+		if (n_na > cap.f || n_x > cap.x) return STEP_SPILL;
+		b.sync();
+		// extend the states that entered a vertex with a match
+		i32 n_f2 = 0;
+		for (i32 base = 0; base < n_na; base += NT) {
+			const i32 j = base + tid;
+			const bool act = j < n_na;
+			const u64 key = act ? w.F1[j] : 0;
 			const u32 v = key_v(key);
 			const i32 d = key_d(key);
 			i32 k = key_k(key);
@@ -366,215 +650,240 @@ __device__ i32 align_read(const DGraph &g, const uint8_t *q, i32 ql, u32 v0, con
 			const i32 vl = (i32)vi.y;
 			const i32 max_k = (ql - d < vl ? ql - d : vl) - 1;
 			k = extend_wave(act, g.seq, vi.x, q, d, k, max_k, &chars);
-			if (act) w.A[j] = mk(v, d, k);
 			const bool qend = act && d + k == ql - 1;
-			if (__ballot(qend)) found = true;
+			found |= qend;
 			const bool vend = act && !qend && k == vl - 1;
-			const i32 slot = wave_slot(vend, &n_f);
-			if (vend && slot < cap.f) w.F0[slot] = mk(v, d, k);
-		}
-		if (found) { *n_chars += chars; return s; }
-		if (n_f > cap.f) { *status = GWF_ST_OVERFLOW; return -1; }
-		wsync();
-
-		// ---- 2. in-vertex successors S (sorted) and exits T1
-		i32 n_s = 0, n_t = 0;
-		bool ovf = false;
-		for (i32 base = 0; base < n_a; base += WAVE) {
-			const i32 j = base + lane;
-			const bool act = j < n_a;
-			const u64 cur = act ? w.A[j] : 0;
-			const u64 prv = (act && j > 0) ? w.A[j - 1] : EMPTY_KEY;
-			const u64 nxt = (act && j + 1 < n_a) ? w.A[j + 1] : EMPTY_KEY;
-			const u32 v = key_v(cur);
-			const i32 d = key_d(cur), k = key_k(cur);
-			const i32 vl = act ? (i32)g.vinfo[v].y : 0;
-			const bool hp = act && j > 0 && key_v(prv) == v;
-			const bool hn = act && j + 1 < n_a && key_v(nxt) == v;
-			const i32 dp = key_d(prv), kp = key_k(prv), dn = key_d(nxt), kn = key_k(nxt);
-			// outputs on diagonals d-1, d, d+1 (max over ins/mis/del candidates)
-			const bool e0 = act && !(hp && dp >= d - 2);
-			const i32 k0 = k + 1;
-			i32 k1 = k + 1;
-			if (hp && dp == d - 1 && kp > k1) k1 = kp;
-			if (hn && dn == d + 1 && kn + 1 > k1) k1 = kn + 1;
-			const bool e1 = act;
-			const bool e2 = act && !(hn && dn == d + 1);
-			i32 k2 = k;
-			if (hn && dn == d + 2 && kn + 1 > k2) k2 = kn + 1;
-			const bool s0 = e0 && (d - 1) + k0 < ql && k0 < vl, t0 = e0 && k0 == vl;
-			const bool s1 = e1 && d + k1 < ql && k1 < vl, t1 = e1 && k1 == vl;
-			const bool s2 = e2 && (d + 1) + k2 < ql && k2 < vl, t2 = e2 && k2 == vl;
-			i32 tot;
-			i32 p = n_s + scan_excl((i32)s0 + (i32)s1 + (i32)s2, &tot);
-			if (n_s + tot > cap.s) ovf = true;
-			else {
-				if (s0) w.S[p++] = mk(v, d - 1, k0);
-				if (s1) w.S[p++] = mk(v, d, k1);
-				if (s2) w.S[p++] = mk(v, d + 1, k2);
+			const bool mid = act && !qend && !vend;
+			const i32 sf = b.slot(vend, &n_f2);
+			if (vend && sf < cap.f) w.F0[sf] = mk(v, d, k);
+			i32 tx;
+			const i32 sx = n_x + b.scan(mid ? 3 : 0, &tx);
+			n_x += tx;
+			if (mid && sx + 2 < cap.x) {
+				w.X[sx] = mk(v, d - 1, k + 1);
+				w.X[sx + 1] = mk(v, d, k + 1);
+				w.X[sx + 2] = mk(v, d + 1, k);
 			}
-			n_s += tot;
-			p = n_t + scan_excl((i32)t0 + (i32)t1 + (i32)t2, &tot);
-			if (n_t + tot > cap.t) ovf = true;
-			else {
-				if (t0) w.T[p++] = diag_id(v, d - 1);
-				if (t1) w.T[p++] = diag_id(v, d);
-				if (t2) w.T[p++] = diag_id(v, d + 1);
-			}
-			n_t += tot;
 		}
-		if (ovf) { *status = GWF_ST_OVERFLOW; return -1; }
-		const i32 n_t1 = n_t;
+		if (b.any(found)) return STEP_FOUND;
+		if (n_f2 > cap.f || n_x > cap.x) return STEP_SPILL;
+		b.sync();
+		n_f = n_f2;
+		closure_origin = true;
+	}
 
-		// ---- 3. vertex-crossing closure
-		for (i32 t = lane; t < VIS_SLOTS; t += WAVE) vis[t] = EMPTY_KEY;
-		wsync();
-		i32 n_x = 0;
-		u64 *fb = w.F0, *nb = w.F1;
-		bool closure_origin = false;
-		while (n_f > 0) {
-			i32 n_na = 0;
-			for (i32 base = 0; base < n_f; base += WAVE) {
-				const i32 j = base + lane;
-				const bool act = j < n_f;
-				const u64 key = act ? fb[j] : 0;
-				const u32 v = key_v(key);
-				const i32 d = key_d(key), k = key_k(key);
-				const i32 i = d + k;
-				const uint4 vi = act ? g.vinfo[v] : make_uint4(0, 0, 0, 0);
-				const bool rec = act && closure_origin;
-				const i32 ts = wave_slot(rec, &n_t);
-				if (rec && ts < cap.t) w.T[ts] = diag_id(v, d);
-				const uint8_t c = act ? q[i + 1] : 0;
-				const i32 deg = act ? (i32)vi.w : 0;
-				i32 maxdeg = deg;
-				for (int o = 32; o > 0; o >>= 1) {
-					const i32 y = __shfl_xor(maxdeg, o);
-					maxdeg = maxdeg > y ? maxdeg : y;
+	// ---- capacity checks for the destructive part of the step (P is free from here on)
+	if (n_s + n_x > cap.m || n_x > cap.m || 4 * n_t1 + 2 * n_h > cap.m || n_h + n_t1 > cap.h) return STEP_SPILL;
+
+	// ---- P5: history H[hs ^ 1] = coalesce(H[hs] u runs of this step's exits)
+	u64 *Hcur_lo = hs ? w.H1lo : w.H0lo, *Hcur_hi = hs ? w.H1hi : w.H0hi;
+	u64 *Hlo = hs ? w.H0lo : w.H1lo, *Hhi = hs ? w.H0hi : w.H1hi;
+	i32 nh2;
+	{
+		u64 *TIlo = w.P, *TIhi = w.P + n_t1;
+		i32 n_ti = 0;
+		for (i32 base = 0; base < n_t1; base += NT) {
+			const i32 j = base + tid;
+			const bool act = j < n_t1;
+			const u64 x = act ? w.T[j] : 0;
+			const bool start = act && (j == 0 || w.T[j - 1] + 1 != x);
+			const bool end = act && (j + 1 == n_t1 || w.T[j + 1] != x + 1);
+			i32 cnt;
+			const i32 e = b.scan(start ? 1 : 0, &cnt);
+			if (start) TIlo[n_ti + e] = x;
+			if (end) TIhi[n_ti + e + (start ? 1 : 0) - 1] = x + 1;
+			n_ti += cnt;
+		}
+		b.sync();
+		u64 *Clo = w.P + 2 * n_t1, *Chi = Clo + (n_h + n_ti);
+		merge_intervals2(Hcur_lo, Hcur_hi, n_h, TIlo, TIhi, n_ti, Clo, Chi, b);
+		nh2 = coalesce(Clo, Chi, n_h + n_ti, Hlo, Hhi, b);
+	}
+
+	// ---- P4: sort the closure successors into P
+	const u64 *Xs;
+	if constexpr (HBM) {
+		sort_hbm(w.X, n_x, w.sort_lds, w.sort_cap, w.sort_tmp, b);
+		Xs = w.X;
+	} else {
+		sort_runs(w.X, n_x, w.P, b);
+		Xs = w.P;
+	}
+
+	// ---- P6: merge the sorted X into Q in place (merge path, right to left; Q first on ties)
+	const i32 n_m = n_s + n_x;
+	if (n_x > 0) {
+		for (i32 hi = n_m; hi > 0; hi -= NT) {
+			const i32 p = hi - NT + tid;
+			u64 val = 0;
+			if (p >= 0) {
+				i32 lo = p - n_x > 0 ? p - n_x : 0, up = p < n_s ? p : n_s;
+				while (lo < up) {
+					const i32 mid = (lo + up) >> 1;
+					if (w.Q[mid] <= Xs[p - 1 - mid]) lo = mid + 1;
+					else up = mid;
 				}
-				i32 n_ext = 0;
-				for (i32 e = 0; e < maxdeg; ++e) {
-					const bool has = e < deg;
-					const u32 wv = has ? g.arc_w[vi.z + e] : 0;
-					const i32 ol = has ? g.arc_o[vi.z + e] : 0;
-					const u32 woff = has ? g.vinfo[wv].x : 0;
-					const bool m = has && g.seq[woff + (u32)ol] == c;
-					n_ext += m;
-					const bool absent = has && vis_insert(vis, (u64)wv << 32 | (u32)(i + 1));
-					const bool pa = absent && m;
-					const i32 sa = wave_slot(pa, &n_na);
-					if (pa && sa < cap.f) nb[sa] = mk(wv, i + 1 - ol, ol);
-					const bool px = absent && !m;
-					const u64 xm = __ballot(px);
-					const i32 sx = n_x + 2 * (i32)rank_below(xm);
-					n_x += 2 * __popcll(xm);
-					if (px && sx + 1 < cap.x) {
-						w.X[sx] = mk(wv, i - ol, ol);
-						w.X[sx + 1] = mk(wv, i + 1 - ol, ol);
-					}
-				}
-				const bool pi = rec && (deg == 0 || n_ext != deg);
-				const i32 si = wave_slot(pi, &n_x);
-				if (pi && si < cap.x) w.X[si] = mk(v, d + 1, k);
+				const i32 i = lo, jx = p - lo;
+				val = (i < n_s && (jx >= n_x || w.Q[i] <= Xs[jx])) ? w.Q[i] : Xs[jx];
 			}
-			if (n_na > cap.f || n_x > cap.x || n_t > cap.t) { *status = GWF_ST_OVERFLOW; return -1; }
-			wsync();
-			// extend the states that entered a vertex with a match
-			i32 n_f2 = 0;
-			for (i32 base = 0; base < n_na; base += WAVE) {
-				const i32 j = base + lane;
-				const bool act = j < n_na;
-				const u64 key = act ? nb[j] : 0;
-				const u32 v = key_v(key);
-				const i32 d = key_d(key);
-				i32 k = key_k(key);
-				const uint4 vi = act ? g.vinfo[v] : make_uint4(0, 0, 0, 0);
-				const i32 vl = (i32)vi.y;
-				const i32 max_k = (ql - d < vl ? ql - d : vl) - 1;
-				k = extend_wave(act, g.seq, vi.x, q, d, k, max_k, &chars);
-				const bool qend = act && d + k == ql - 1;
-				if (__ballot(qend)) found = true;
-				const bool vend = act && !qend && k == vl - 1;
-				const bool mid = act && !qend && !vend;
-				const i32 sf = wave_slot(vend, &n_f2);
-				if (vend && sf < cap.f) fb[sf] = mk(v, d, k);
-				const u64 xm = __ballot(mid);
-				const i32 sx = n_x + 3 * (i32)rank_below(xm);
-				n_x += 3 * __popcll(xm);
-				if (mid && sx + 2 < cap.x) {
-					w.X[sx] = mk(v, d - 1, k + 1);
-					w.X[sx + 1] = mk(v, d, k + 1);
-					w.X[sx + 2] = mk(v, d + 1, k);
+			b.sync();
+			if (p >= 0) w.Q[p] = val;
+			b.sync();
+		}
+	}
+
+	// ---- P7: furthest state per diagonal, outside the forbidden intervals
+	i32 n_new = 0;
+	for (i32 base = 0; base < n_m; base += NT) {
+		const i32 j = base + tid;
+		const bool act = j < n_m;
+		const u64 x = act ? w.Q[j] : 0;
+		const u64 dg = key_diag(x);
+		const bool first = act && (j == 0 || key_diag(w.Q[j - 1]) != dg);
+		const bool keep = first && !forbidden(Hlo, Hhi, nh2, dg);
+		const i32 sl = b.slot(keep, &n_new);
+		if (keep) w.P[sl] = x;
+	}
+	b.sync();
+	n_a = n_new;
+	n_h = nh2;
+	hs ^= 1;
+	return n_a == 0 ? STEP_EMPTY : STEP_NEXT;
+}
+
+// Copy the step input (wavefront + history) from LDS to HBM buffers.
+template <int NW>
+__device__ __forceinline__ void spill_state(const Bufs &from, const Bufs &to, i32 n_a, i32 n_h, i32 hs, const Blk<NW> &b) {
+	for (i32 t = (i32)b.tid(); t < n_a; t += Blk<NW>::NT) to.P[t] = from.P[t];
+	const u64 *flo = hs ? from.H1lo : from.H0lo, *fhi = hs ? from.H1hi : from.H0hi;
+	u64 *tlo = hs ? to.H1lo : to.H0lo, *thi = hs ? to.H1hi : to.H0hi;
+	for (i32 t = (i32)b.tid(); t < n_h; t += Blk<NW>::NT) {
+		tlo[t] = flo[t];
+		thi[t] = fhi[t];
+	}
+	b.sync();
+}
+
+struct ReadOut {
+	i32 ed, status;
+	int64_t steps, states, chars, spill_step;
+};
+
+// Align one read: LDS-resident while it fits (USE_LDS), then in HBM.
+template <int NW, bool USE_LDS>
+__device__ __forceinline__ ReadOut align_read(const DGraph &g, const uint8_t *q, i32 ql, u32 v0, const Bufs &lds, const Bufs &hbm,
+                              const Blk<NW> &b, u64 *vis) {
+	ReadOut o;
+	o.ed = -1, o.status = GWF_ST_OK, o.steps = 0, o.states = 0, o.chars = 0, o.spill_step = -1;
+	i32 n_a = 1, n_h = 0, hs = 0;
+	int64_t chars = 0;
+	bool in_lds = USE_LDS;
+	if (b.tid() == 0) (in_lds ? lds.P : hbm.P)[0] = mk(v0, 0, -1);
+	b.sync();
+	for (i32 s = 0;; ++s) {
+		o.steps += 1;
+		o.states += n_a;
+		if (s > ql + 1) { o.status = GWF_ST_LIMIT; break; }
+		int rc = STEP_SPILL;
+		if constexpr (USE_LDS) {
+			if (in_lds) {
+				rc = gwf_step<NW, false>(g, q, ql, lds, b, vis, n_a, n_h, hs, chars);
+				if (rc == STEP_SPILL) {  // the step input is intact: move it to HBM and redo the step there
+					spill_state(lds, hbm, n_a, n_h, hs, b);
+					in_lds = false;
+					o.spill_step = s;
 				}
 			}
-			if (found) { *n_chars += chars; return s; }
-			if (n_f2 > cap.f || n_x > cap.x) { *status = GWF_ST_OVERFLOW; return -1; }
-			wsync();
-			n_f = n_f2;
-			closure_origin = true;
 		}
+		if (rc == STEP_SPILL) rc = gwf_step<NW, true>(g, q, ql, hbm, b, vis, n_a, n_h, hs, chars);
+		if (rc == STEP_SPILL) { o.status = GWF_ST_OVERFLOW; break; }
+		if (rc == STEP_FOUND) { o.ed = s; break; }
+		if (rc == STEP_EMPTY) { o.ed = -1; break; }
+	}
+	o.chars = chars;  // per thread; summed in write_out
+	return o;
+}
 
-		// ---- 4. sort the closure outputs and this step's closure exits
-		sort_keys(w.X, n_x, lds_sort, w.SG);
-		sort_keys(w.T + n_t1, n_t - n_t1, lds_sort, w.SG);
-
-		// ---- 5. merge in-vertex and closure successors
-		merge_keys(w.S, n_s, w.X, n_x, w.M);
-		const i32 n_m = n_s + n_x;
-
-		// ---- 6. forbidden-interval history: H2 = coalesce(H u T)
-		merge_keys(w.T, n_t1, w.T + n_t1, n_t - n_t1, w.Tb);
-		merge_intervals(Hlo, Hhi, n_h, w.Tb, n_t, w.Clo, w.Chi);
-		const i32 nh2 = coalesce(w.Clo, w.Chi, n_h + n_t, Hlo2, Hhi2, cap.h);
-		if (nh2 < 0) { *status = GWF_ST_OVERFLOW; return -1; }
-		{ u64 *t = Hlo; Hlo = Hlo2; Hlo2 = t; t = Hhi; Hhi = Hhi2; Hhi2 = t; }
-		n_h = nh2;
-
-		// ---- 7. keep the furthest state per diagonal, drop forbidden ones
-		i32 n_new = 0;
-		for (i32 base = 0; base < n_m; base += WAVE) {
-			const i32 j = base + lane;
-			const bool act = j < n_m;
-			const u64 x = act ? w.M[j] : 0;
-			const u64 dg = key_diag(x);
-			const bool first = act && (j == 0 || key_diag(w.M[j - 1]) != dg);
-			const bool keep = first && !forbidden(Hlo, Hhi, n_h, dg);
-			const i32 slot = wave_slot(keep, &n_new);
-			if (keep && slot < cap.a) w.A[slot] = x;
-		}
-		if (n_new > cap.a) { *status = GWF_ST_OVERFLOW; return -1; }
-		wsync();
-		n_a = n_new;
-		if (n_a == 0) { *n_chars += chars; return -1; }
+template <int NW>
+__device__ __forceinline__ void write_out(const ReadOut &o, i32 r, u64 t0, i32 *out_ed, i32 *out_status, int64_t *out_counts,
+                          const Blk<NW> &b) {
+	int64_t c = o.chars;
+#pragma unroll
+	for (int s = 32; s > 0; s >>= 1) c += __shfl_xor(c, s);
+	if constexpr (NW > 1) {
+		if (lane_id() == 0) b.redu[b.wid()] = (u64)c;
+		b.sync();
+		c = 0;
+		for (int w = 0; w < NW; ++w) c += (int64_t)b.redu[w];
+		b.sync();
+	}
+	if (b.tid() == 0) {
+		int64_t *cc = out_counts + GWF_NCOUNT * (size_t)r;
+		out_ed[r] = o.ed;
+		out_status[r] = o.status;
+		cc[0] = o.steps;
+		cc[1] = o.states;
+		cc[2] = c;
+		cc[3] = (int64_t)t0;
+		cc[4] = (int64_t)wall_clock64();
+		cc[5] = o.spill_step;
 	}
 }
 
 // ------------------------------------------------------------ kernels
-__global__ __launch_bounds__(WAVE) void gwf_plain_kernel(DGraph g, DReads rd, const i32 *read_idx, i32 n,
-                                                         Caps caps, u64 *scratch, u64 ws_elems_per_read,
-                                                         i32 *out_ed, i32 *out_status, int64_t *out_counts) {
-	__shared__ u64 lds_sort[SORT_LDS];
+// Plain: one wavefront per read, grid = reads, every array in HBM.
+__global__ __launch_bounds__(WAVE) void gwf_plain_kernel(DGraph g, DReads rd, const i32 *read_idx, i32 n, Caps caps,
+                                                         u64 *scratch, u64 ws_elems_per_read, i32 *out_ed,
+                                                         i32 *out_status, int64_t *out_counts) {
+	__shared__ u64 lds_sort[SORT_CHUNK];
 	__shared__ u64 vis[VIS_SLOTS];
-	const i32 b = blockIdx.x;
-	if (b >= n) return;
-	const i32 r = read_idx ? read_idx[b] : b;
-	Ws w = carve(scratch + (size_t)b * ws_elems_per_read, caps);
+	__shared__ i32 red[1];
+	__shared__ u64 redu[1];
+	const i32 blk = blockIdx.x;
+	if (blk >= n) return;
+	const i32 r = read_idx ? read_idx[blk] : blk;
+	Blk<1> b{red, redu};
+	Bufs w = carve(scratch + (size_t)blk * ws_elems_per_read, caps, true);
+	w.sort_lds = lds_sort;
+	w.sort_cap = SORT_CHUNK;
 	const u64 t0 = wall_clock64();
-	int64_t steps = 0, states = 0, chars = 0;
-	i32 st;
-	const i32 ed = align_read(g, rd.seq + rd.off[r], rd.ql[r], (u32)rd.v0[r], caps, w, lds_sort, vis, &st,
-	                          &steps, &states, &chars);
-	// chars is per lane; reduce
-	for (int o = 32; o > 0; o >>= 1) chars += __shfl_xor(chars, o);
-	if (lane_id() == 0) {
-		int64_t *c = out_counts + GWF_NCOUNT * (size_t)r;
-		out_ed[r] = ed;
-		out_status[r] = st;
-		c[0] = steps;
-		c[1] = states;
-		c[2] = chars;
-		c[3] = (int64_t)t0;
-		c[4] = (int64_t)wall_clock64();
+	const ReadOut o = align_read<1, false>(g, rd.seq + rd.off[r], rd.ql[r], (u32)rd.v0[r], w, w, b, vis);
+	write_out<1>(o, r, t0, out_ed, out_status, out_counts, b);
+}
+
+// Persistent: resident workgroups pull reads from a queue; each read is
+// aligned by NW waves with its arrays in LDS until they outgrow it.
+template <int NW>
+__global__ __launch_bounds__(WAVE *NW) void gwf_persistent_kernel(DGraph g, DReads rd, const i32 *order, i32 n,
+                                                                 i32 *head, Caps lc, Caps hc, u64 *scratch,
+                                                                 u64 ws_elems_per_wg, i32 *out_ed, i32 *out_status,
+                                                                 int64_t *out_counts) {
+	extern __shared__ u64 arena[];
+	__shared__ u64 vis[VIS_SLOTS];
+	__shared__ i32 red[NW];
+	__shared__ u64 redu[NW];
+	__shared__ i32 next_q;
+	Blk<NW> b{red, redu};
+	const Bufs lb = carve(arena, lc, false);
+	Bufs hb = carve(scratch + (size_t)blockIdx.x * ws_elems_per_wg, hc, true);
+	hb.sort_lds = arena;  // the LDS arena is free once a read has spilled
+	{
+		const size_t le = lds_elems(lc);
+		i32 sc = 2;
+		while ((size_t)sc * 2 <= le && sc < (1 << 14)) sc <<= 1;
+		hb.sort_cap = sc;
+	}
+	for (;;) {
+		if (b.tid() == 0) next_q = atomicAdd(head, 1);
+		b.sync();
+		const i32 qi = next_q;
+		b.sync();
+		if (qi >= n) break;
+		const i32 r = order[qi];
+		const u64 t0 = wall_clock64();
+		const ReadOut o = align_read<NW, true>(g, rd.seq + rd.off[r], rd.ql[r], (u32)rd.v0[r], lb, hb, b, vis);
+		write_out<NW>(o, r, t0, out_ed, out_status, out_counts, b);
+		b.sync();
 	}
 }
 
@@ -585,4 +894,53 @@ hipError_t gwf_launch_plain(const DGraph &g, const DReads &r, const i32 *read_id
 	hipLaunchKernelGGL(gwf_plain_kernel, dim3(n), dim3(WAVE), 0, st, g, r, read_idx, n, caps, scratch,
 	                   (u64)(ws_bytes / 8), out_ed, out_status, out_counts);
 	return hipGetLastError();
+}
+
+template <int NW>
+static hipError_t launch_persist(const DGraph &g, const DReads &r, const i32 *order, i32 n, i32 *head,
+                                 const PersistCfg &cfg, u64 *scratch, size_t ws_bytes, i32 *out_ed, i32 *out_status,
+                                 int64_t *out_counts, hipStream_t st) {
+	static bool attr_set = false;
+	if (!attr_set) {
+		hipError_t e = hipFuncSetAttribute((const void *)gwf_persistent_kernel<NW>,
+		                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)gwf_max_lds_bytes());
+		if (e != hipSuccess) return e;
+		attr_set = true;
+	}
+	hipLaunchKernelGGL(gwf_persistent_kernel<NW>, dim3(cfg.wgs), dim3(WAVE * NW), cfg.lds_bytes, st, g, r, order, n,
+	                   head, cfg.lds, cfg.hbm, scratch, (u64)(ws_bytes / 8), out_ed, out_status, out_counts);
+	return hipGetLastError();
+}
+
+hipError_t gwf_launch_persistent(const DGraph &g, const DReads &r, const i32 *order, i32 n, i32 *head,
+                                 const PersistCfg &cfg, u64 *scratch, size_t ws_bytes, i32 *out_ed, i32 *out_status,
+                                 int64_t *out_counts, hipStream_t st) {
+	if (n <= 0) return hipSuccess;
+	switch (cfg.nw) {
+	case 1: return launch_persist<1>(g, r, order, n, head, cfg, scratch, ws_bytes, out_ed, out_status, out_counts, st);
+	case 2: return launch_persist<2>(g, r, order, n, head, cfg, scratch, ws_bytes, out_ed, out_status, out_counts, st);
+	case 4: return launch_persist<4>(g, r, order, n, head, cfg, scratch, ws_bytes, out_ed, out_status, out_counts, st);
+	default: return hipErrorInvalidValue;
+	}
+}
+
+size_t gwf_max_lds_bytes() { return 160 * 1024 - 4096; }
+
+template <int NW>
+static int occ(size_t lds_bytes) {
+	int nb = 0;
+	(void)hipFuncSetAttribute((const void *)gwf_persistent_kernel<NW>, hipFuncAttributeMaxDynamicSharedMemorySize,
+	                          (int)gwf_max_lds_bytes());
+	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, gwf_persistent_kernel<NW>, WAVE * NW, lds_bytes) != hipSuccess)
+		return 1;
+	return nb;
+}
+
+int gwf_persist_occupancy(int nw, size_t lds_bytes) {
+	switch (nw) {
+	case 1: return occ<1>(lds_bytes);
+	case 2: return occ<2>(lds_bytes);
+	case 4: return occ<4>(lds_bytes);
+	default: return 1;
+	}
 }

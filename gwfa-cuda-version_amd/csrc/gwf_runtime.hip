@@ -3,6 +3,7 @@
 // re-runs of reads that outgrow their workspace, and the host-callable gwf_ed.
 #include <algorithm>
 #include <chrono>
+#include <functional>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -38,6 +39,7 @@ void set_err(const char *fmt, ...) {
 	} while (0)
 
 constexpr size_t kSeqPad = 2048;  // bytes readable past the end of packed sequences
+constexpr i32 kProbeLen = 256;    // prefix aligned to estimate each read's error rate
 
 int device_ready() {
 	int n = 0;
@@ -58,6 +60,7 @@ struct gwf_dgraph_s {
 
 struct gwf_dreads_s {
 	DReads r;
+	const i32 *ql_probe = nullptr;  // device: min(ql, kProbeLen) per read
 	void *mem = nullptr;
 	std::vector<i32> ql, v0;
 };
@@ -71,9 +74,10 @@ struct RunCtx {
 	std::mutex mu;
 	int dev = -1;
 	int wall_khz = 0;
+	int n_cu = 0;
 	hipStream_t s = nullptr;
 	hipEvent_t e0 = nullptr, e1 = nullptr;
-	i32 *ed = nullptr, *status = nullptr, *idx = nullptr;
+	i32 *ed = nullptr, *status = nullptr, *idx = nullptr, *head = nullptr;
 	int64_t *cnt = nullptr;
 	size_t n_cap = 0;
 	u64 *scratch = nullptr;
@@ -96,7 +100,7 @@ int ctx_prepare(RunCtx &c, size_t n) {
 	int dev = 0;
 	HIPCHK(hipGetDevice(&dev));
 	if (c.dev != dev) {  // first use (device buffers of another device are dropped)
-		c.ed = c.status = c.idx = nullptr;
+		c.ed = c.status = c.idx = c.head = nullptr;
 		c.cnt = nullptr;
 		c.scratch = nullptr;
 		c.n_cap = c.scratch_bytes = 0;
@@ -104,6 +108,8 @@ int ctx_prepare(RunCtx &c, size_t n) {
 		HIPCHK(hipEventCreate(&c.e0));
 		HIPCHK(hipEventCreate(&c.e1));
 		HIPCHK(hipDeviceGetAttribute(&c.wall_khz, hipDeviceAttributeWallClockRate, dev));
+		HIPCHK(hipDeviceGetAttribute(&c.n_cu, hipDeviceAttributeMultiprocessorCount, dev));
+		HIPCHK(hipMalloc((void **)&c.head, 256));
 		c.dev = dev;
 	}
 	if (c.n_cap < n) {
@@ -119,27 +125,62 @@ int ctx_prepare(RunCtx &c, size_t n) {
 	return GWF_OK;
 }
 
-std::mutex g_cache_mu;
-std::unordered_map<const gwf_graph_t *, gwf_dgraph_t *> g_cache;
+int64_t env_int(const char *name, int64_t dflt) {
+	const char *v = getenv(name);
+	return v && *v ? atoll(v) : dflt;
+}
 
-// Initial workspace: the reference's per-read sizing (main.cu:498-504) plus
-// headroom.  GWF_DEBUG_CAP_A overrides it (tests use it to force the
-// overflow -> re-run path).
+// HBM workspace of a read of length ql: the reference's per-read sizing
+// (main.cu:498-504) plus headroom, times `scale` after overflows.
+// GWF_DEBUG_CAP_A overrides the base (tests force the re-run path with it).
 Caps caps_for(i32 ql, int scale) {
-	Caps c;
-	const char *dbg_env = getenv("GWF_DEBUG_CAP_A");
-	const int64_t dbg = dbg_env ? atoll(dbg_env) : 0;
+	const int64_t dbg = env_int("GWF_DEBUG_CAP_A", 0);
 	int64_t a = dbg > 0 ? dbg : (int64_t)(1.6 * ql) + 256;
 	a *= scale;
-	if (a > (1 << 26)) a = 1 << 26;
-	c.a = (i32)a;
-	c.s = (i32)std::min<int64_t>(3 * a + 8, INT32_MAX / 4);
+	if (a > (1 << 25)) a = 1 << 25;
+	Caps c;
+	c.m = (i32)(2 * a);
 	c.x = (i32)a;
 	c.f = (i32)a;
 	c.t = (i32)a;
 	c.h = (i32)a;
 	return c;
 }
+
+// LDS capacities for a wavefront of up to ~m states.
+Caps lds_caps(i32 m) {
+	Caps c;
+	c.m = m;
+	c.x = std::max(64, (m / 2 + 63) / 64 * 64);
+	c.f = std::max(64, m / 4);
+	c.t = std::max(64, m / 2);
+	c.h = std::max(64, m / 4);
+	return c;
+}
+
+// The tuning table: waves per read and LDS wavefront capacity for a bin of
+// reads, from the largest expected wavefront of the bin.  Overridable with
+// GWF_PERSIST_NW / GWF_PERSIST_M.
+PersistCfg choose_cfg(const RunCtx &cx, int64_t est_states, i32 max_ql) {
+	PersistCfg c;
+	// Measured on MI355X (config 2 sweep): LDS capacity beyond ~1024 states
+	// costs more in resident reads than it saves in HBM spills.
+	i32 m = est_states <= 192 ? 256 : est_states <= 512 ? 512 : 1024;
+	i32 nw = m <= 256 ? 1 : m <= 512 ? 2 : 4;
+	nw = (i32)env_int("GWF_PERSIST_NW", nw);
+	m = (i32)env_int("GWF_PERSIST_M", m);
+	c.nw = nw;
+	c.lds = lds_caps(m);
+	c.lds_bytes = gwf_lds_bytes(c.lds);
+	c.hbm = caps_for(max_ql, 1);
+	const int per_cu = gwf_persist_occupancy(nw, c.lds_bytes);
+	c.wgs = std::max(1, per_cu) * std::max(1, cx.n_cu);
+	c.wgs = (i32)env_int("GWF_PERSIST_WGS", c.wgs);
+	return c;
+}
+
+std::mutex g_cache_mu;
+std::unordered_map<const gwf_graph_t *, gwf_dgraph_t *> g_cache;
 
 }  // namespace
 
@@ -197,8 +238,12 @@ int gwf_dgraph_upload(const gwf_graph_t *g, gwf_dgraph_t **out) {
 		total += g->len[v];
 	}
 	if (total + kSeqPad >= (1ull << 32)) { set_err("gwf_dgraph_upload: graph sequence exceeds 4 GiB"); return GWF_ERR_ARG; }
-	std::vector<u32> aw(na ? na : 1);
-	std::vector<i32> ao(na ? na : 1);
+	std::vector<uint4> arcs(na ? na : 1, make_uint4(0, 0, 0, 0));
+	std::vector<u32> seq_off(nv);
+	{
+		uint64_t o = 0;
+		for (u32 v = 0; v < nv; ++v) seq_off[v] = (u32)o, o += g->len[v];
+	}
 	for (u32 i = 0; i < na; ++i) {
 		const u32 u = (u32)(g->arc[i].a >> 32), w = (u32)g->arc[i].a;
 		const i32 o = g->arc[i].o;
@@ -212,7 +257,7 @@ int gwf_dgraph_upload(const gwf_graph_t *g, gwf_dgraph_t **out) {
 			return GWF_ERR_ARG;
 		}
 		in_ol[w] = o;
-		aw[i] = w, ao[i] = o;
+		arcs[i] = make_uint4(w, (u32)o | (u32)(uint8_t)g->seq[w][o] << 24, seq_off[w], g->len[w]);
 	}
 	uint64_t so = 0;
 	for (u32 v = 0; v < nv; ++v) {
@@ -220,10 +265,9 @@ int gwf_dgraph_upload(const gwf_graph_t *g, gwf_dgraph_t **out) {
 		vinfo[v] = make_uint4((u32)so, g->len[v], (u32)(ax >> 32), (u32)ax);
 		so += g->len[v];
 	}
-	const size_t b_vi = sizeof(uint4) * nv, b_seq = (size_t)total + kSeqPad, b_aw = 4 * (size_t)aw.size(),
-	             b_ao = 4 * (size_t)ao.size();
+	const size_t b_vi = sizeof(uint4) * nv, b_seq = (size_t)total + kSeqPad, b_arc = sizeof(uint4) * arcs.size();
 	auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-	const size_t bytes = al(b_vi) + al(b_seq) + al(b_aw) + al(b_ao);
+	const size_t bytes = al(b_vi) + al(b_seq) + al(b_arc);
 	std::vector<uint8_t> seqh(b_seq, 0);
 	so = 0;
 	for (u32 v = 0; v < nv; ++v) {
@@ -243,11 +287,8 @@ int gwf_dgraph_upload(const gwf_graph_t *g, gwf_dgraph_t **out) {
 	dg->g.seq = p;
 	if (e == hipSuccess) e = hipMemcpy(p, seqh.data(), b_seq, hipMemcpyHostToDevice);
 	p += al(b_seq);
-	dg->g.arc_w = (const u32 *)p;
-	if (e == hipSuccess) e = hipMemcpy(p, aw.data(), b_aw, hipMemcpyHostToDevice);
-	p += al(b_aw);
-	dg->g.arc_o = (const i32 *)p;
-	if (e == hipSuccess) e = hipMemcpy(p, ao.data(), b_ao, hipMemcpyHostToDevice);
+	dg->g.arc = (const uint4 *)p;
+	if (e == hipSuccess) e = hipMemcpy(p, arcs.data(), b_arc, hipMemcpyHostToDevice);
 	dg->g.n_vtx = nv;
 	dg->n_vtx = nv;
 	if (e != hipSuccess) {
@@ -291,7 +332,7 @@ int gwf_dreads_upload(int32_t n, const char *seqs, const int64_t *off, const int
 	for (i32 i = 0; i < n; ++i) memcpy(&buf[noff[i]], seqs + off[i], ql[i]);
 	auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
 	const size_t b_seq = buf.size(), b_off = 8 * noff.size(), b_ql = 4 * (size_t)(n ? n : 1);
-	const size_t bytes = al(b_seq) + al(b_off) + 2 * al(b_ql);
+	const size_t bytes = al(b_seq) + al(b_off) + 3 * al(b_ql);
 	if (hipMalloc(&dr->mem, bytes) != hipSuccess) {
 		delete dr;
 		set_err("gwf_dreads_upload: hipMalloc(%zu) failed", bytes);
@@ -309,6 +350,13 @@ int gwf_dreads_upload(int32_t n, const char *seqs, const int64_t *off, const int
 	p += al(b_ql);
 	if (e == hipSuccess && n) e = hipMemcpy(p, dr->v0.data(), 4 * (size_t)n, hipMemcpyHostToDevice);
 	dr->r.v0 = (const i32 *)p;
+	p += al(b_ql);
+	{
+		std::vector<i32> qp(n ? n : 1);
+		for (i32 i = 0; i < n; ++i) qp[i] = std::min(dr->ql[i], kProbeLen);
+		if (e == hipSuccess && n) e = hipMemcpy(p, qp.data(), 4 * (size_t)n, hipMemcpyHostToDevice);
+	}
+	dr->ql_probe = (const i32 *)p;
 	dr->r.n = n;
 	if (e != hipSuccess) {
 		set_err("gwf_dreads_upload: hipMemcpy failed: %s", hipGetErrorString(e));
@@ -325,6 +373,81 @@ void gwf_dreads_free(gwf_dreads_t *dr) {
 	delete dr;
 }
 
+namespace {
+
+struct RunAcc {  // statistics of one gwf_ed_run call
+	gwf_run_stats_t st;
+	hipError_t err = hipSuccess;
+	const char *what = "";
+};
+
+#define ACC(x)                                              \
+	do {                                                    \
+		hipError_t e_ = (x);                                \
+		if (e_ != hipSuccess) {                             \
+			acc.err = e_;                                   \
+			acc.what = #x;                                  \
+			return false;                                   \
+		}                                                   \
+	} while (0)
+
+bool timed_launch(RunCtx &cx, RunAcc &acc, const std::function<hipError_t()> &launch) {
+	ACC(hipEventRecord(cx.e0, cx.s));
+	ACC(launch());
+	ACC(hipEventRecord(cx.e1, cx.s));
+	ACC(hipEventSynchronize(cx.e1));
+	float ms = 0;
+	ACC(hipEventElapsedTime(&ms, cx.e0, cx.e1));
+	acc.st.kernel_ms += ms;
+	acc.st.n_launch += 1;
+	return true;
+}
+
+// Plain kernel over `todo` in batches that fit the HBM budget.
+bool run_plain(RunCtx &cx, RunAcc &acc, const gwf_dgraph_t *dg, const DReads &r, const std::vector<i32> &qlh,
+               const std::vector<i32> &todo, int scale, size_t budget) {
+	size_t pos = 0;
+	while (pos < todo.size()) {
+		i32 maxql = 0;
+		size_t end = pos, ws = 0;
+		while (end < todo.size()) {
+			const i32 q = std::max(maxql, qlh[todo[end]]);
+			const size_t wsb = gwf_ws_bytes(caps_for(q, scale));
+			if (end > pos && wsb * (end - pos + 1) > budget) break;
+			maxql = q, ws = wsb, ++end;
+		}
+		const Caps caps = caps_for(maxql, scale);
+		const i32 nb = (i32)(end - pos);
+		ACC(grow(&cx.scratch, &cx.scratch_bytes, ws * nb));
+		ACC(hipMemcpyAsync(cx.idx, todo.data() + pos, 4 * (size_t)nb, hipMemcpyHostToDevice, cx.s));
+		if (!timed_launch(cx, acc, [&] {
+			    return gwf_launch_plain(dg->g, r, cx.idx, nb, caps, cx.scratch, ws, cx.ed, cx.status, cx.cnt, cx.s);
+		    }))
+			return false;
+		pos = end;
+	}
+	return true;
+}
+
+// Persistent kernel over `order` (a queue drained by the resident workgroups).
+bool run_persistent(RunCtx &cx, RunAcc &acc, const gwf_dgraph_t *dg, const DReads &r, const std::vector<i32> &order,
+                    const PersistCfg &cfg) {
+	const i32 n = (i32)order.size();
+	if (n == 0) return true;
+	const size_t ws = gwf_ws_bytes(cfg.hbm);
+	ACC(grow(&cx.scratch, &cx.scratch_bytes, ws * (size_t)cfg.wgs));
+	ACC(hipMemcpyAsync(cx.idx, order.data(), 4 * (size_t)n, hipMemcpyHostToDevice, cx.s));
+	ACC(hipMemsetAsync(cx.head, 0, sizeof(i32), cx.s));
+	return timed_launch(cx, acc, [&] {
+		return gwf_launch_persistent(dg->g, r, cx.idx, n, cx.head, cfg, cx.scratch, ws, cx.ed, cx.status, cx.cnt,
+		                             cx.s);
+	});
+}
+
+#undef ACC
+
+}  // namespace
+
 int gwf_ed_run(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t v1, uint32_t max_lag, int32_t kernel, int32_t *out,
                gwf_run_stats_t *stats) {
 	return gwf_ed_run_timed(dg, dr, v1, max_lag, kernel, out, stats, nullptr, nullptr);
@@ -336,95 +459,109 @@ int gwf_ed_run_timed(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t v1, uint32_t ma
 	const auto t0 = std::chrono::steady_clock::now();
 	if (!dg || !dr || (!out && dr->r.n > 0)) { set_err("gwf_ed_run: NULL argument"); return GWF_ERR_ARG; }
 	if (v1 >= 0) { set_err("gwf_ed_run: v1 >= 0 is not supported on the GPU path"); return GWF_ERR_ARG; }
-	if (kernel != GWF_KERNEL_PLAIN) { set_err("gwf_ed_run: kernel %d is not available", kernel); return GWF_ERR_ARG; }
+	if (kernel != GWF_KERNEL_PLAIN && kernel != GWF_KERNEL_PERSISTENT) {
+		set_err("gwf_ed_run: unknown kernel %d", kernel);
+		return GWF_ERR_ARG;
+	}
 	const i32 n = dr->r.n;
 	for (i32 i = 0; i < n; ++i)
 		if ((u32)dr->v0[i] >= dg->n_vtx) { set_err("gwf_ed_run: read %d has v0 %d >= n_vtx %u", i, dr->v0[i], dg->n_vtx); return GWF_ERR_ARG; }
-	gwf_run_stats_t st;
-	memset(&st, 0, sizeof(st));
-	if (n == 0) { if (stats) *stats = st; return GWF_OK; }
+	RunAcc acc;
+	memset(&acc.st, 0, sizeof(acc.st));
+	if (n == 0) { if (stats) *stats = acc.st; return GWF_OK; }
 
 	RunCtx &cx = g_run;
 	std::lock_guard<std::mutex> lock(cx.mu);
 	int rc = ctx_prepare(cx, (size_t)n);
 	if (rc) return rc;
-	hipStream_t s = cx.s;
 	std::vector<i32> status(n);
 	std::vector<int64_t> cnt(GWF_NCOUNT * (size_t)n);
-	size_t free_b = 0, tot_b = 0, budget = 0;
-	std::vector<i32> todo(n);
-	for (i32 i = 0; i < n; ++i) todo[i] = i;
+	size_t free_b = 0, tot_b = 0;
+	if (hipMemGetInfo(&free_b, &tot_b) != hipSuccess) { set_err("hipMemGetInfo failed"); return GWF_ERR_HIP; }
+	const size_t budget = std::min<size_t>((free_b + cx.scratch_bytes) / 2, (size_t)32 << 30);
+	std::vector<i32> todo;
 	int scale = 1;
+	bool ok = true;
 
-#define RUNCHK(x)                                                                  \
-	do {                                                                           \
-		hipError_t e_ = (x);                                                       \
-		if (e_ != hipSuccess) {                                                    \
-			set_err("%s failed: %s", #x, hipGetErrorString(e_));                   \
-			rc = GWF_ERR_HIP;                                                      \
-			goto done;                                                             \
-		}                                                                          \
-	} while (0)
-
-	RUNCHK(hipMemGetInfo(&free_b, &tot_b));
-	budget = std::min<size_t>((free_b + cx.scratch_bytes) / 2, (size_t)32 << 30);
-
-	while (!todo.empty()) {
-		// batches of consecutive pending reads whose workspaces fit the budget
-		size_t pos = 0;
-		while (pos < todo.size()) {
-			i32 maxql = 0;
-			size_t end = pos, ws = 0;
-			while (end < todo.size()) {
-				const i32 q = std::max(maxql, dr->ql[todo[end]]);
-				const size_t wsb = gwf_ws_bytes(caps_for(q, scale));
-				if (end > pos && wsb * (end - pos + 1) > budget) break;
-				maxql = q, ws = wsb, ++end;
-			}
-			const Caps caps = caps_for(maxql, scale);
-			const i32 nb = (i32)(end - pos);
-			RUNCHK(grow(&cx.scratch, &cx.scratch_bytes, ws * nb));
-			RUNCHK(hipMemcpyAsync(cx.idx, todo.data() + pos, 4 * (size_t)nb, hipMemcpyHostToDevice, s));
-			RUNCHK(hipEventRecord(cx.e0, s));
-			RUNCHK(gwf_launch_plain(dg->g, dr->r, cx.idx, nb, caps, cx.scratch, ws, cx.ed, cx.status, cx.cnt, s));
-			RUNCHK(hipEventRecord(cx.e1, s));
-			RUNCHK(hipEventSynchronize(cx.e1));
-			float ms = 0;
-			RUNCHK(hipEventElapsedTime(&ms, cx.e0, cx.e1));
-			st.kernel_ms += ms;
-			st.n_launch += 1;
-			pos = end;
+	if (kernel == GWF_KERNEL_PERSISTENT) {
+		// probe: align the first kProbeLen bases of every read to estimate its error rate
+		std::vector<i32> ident(n), probe(n);
+		for (i32 i = 0; i < n; ++i) ident[i] = i;
+		DReads rp = dr->r;
+		rp.ql = dr->ql_probe;
+		i32 maxql = 0;
+		for (i32 q : dr->ql) maxql = std::max(maxql, q);
+		PersistCfg pc = choose_cfg(cx, 64, std::min(maxql, kProbeLen));
+		ok = run_persistent(cx, acc, dg, rp, ident, pc);
+		if (ok) ok = hipMemcpy(probe.data(), cx.ed, 4 * (size_t)n, hipMemcpyDeviceToHost) == hipSuccess;
+		if (ok) ok = hipMemcpy(status.data(), cx.status, 4 * (size_t)n, hipMemcpyDeviceToHost) == hipSuccess;
+		// cost ~ ql + c * ED^2 (the wavefront of score s holds ~2s+1 diagonals)
+		std::vector<double> cost(n);
+		int64_t est_max = 0;
+		for (i32 i = 0; i < n && ok; ++i) {
+			const i32 pl = std::min(dr->ql[i], kProbeLen);
+			const double e = (status[i] == GWF_ST_OK && pl > 0) ? (double)std::max(probe[i], 0) / pl : 0.3;
+			const double ed = e * dr->ql[i];
+			cost[i] = dr->ql[i] + 4.0 * ed * ed;
+			est_max = std::max<int64_t>(est_max, (int64_t)(3.2 * ed) + 16);
 		}
-		RUNCHK(hipMemcpy(status.data(), cx.status, 4 * (size_t)n, hipMemcpyDeviceToHost));
+		std::vector<i32> order(ident);
+		std::stable_sort(order.begin(), order.end(), [&](i32 x, i32 y) { return cost[x] > cost[y]; });
+		if (ok) {
+			const PersistCfg cfg = choose_cfg(cx, est_max, maxql);
+			ok = run_persistent(cx, acc, dg, dr->r, order, cfg);
+		}
+		if (ok) ok = hipMemcpy(status.data(), cx.status, 4 * (size_t)n, hipMemcpyDeviceToHost) == hipSuccess;
+		for (i32 i = 0; i < n && ok; ++i)
+			if (status[i] == GWF_ST_OVERFLOW) todo.push_back(i);
+			else if (status[i] != GWF_ST_OK) { set_err("gwf_ed_run: read %d ended with internal status %d", i, status[i]); rc = GWF_ERR_STATE; }
+		if (!todo.empty()) {
+			acc.st.n_retry += (int64_t)todo.size();
+			scale = 4;
+		}
+	} else {
+		todo.resize(n);
+		for (i32 i = 0; i < n; ++i) todo[i] = i;
+	}
+	// plain kernel (and re-runs of reads that overflowed their HBM workspace)
+	while (ok && rc == GWF_OK && !todo.empty()) {
+		ok = run_plain(cx, acc, dg, dr->r, dr->ql, todo, scale, budget);
+		if (!ok) break;
+		ok = hipMemcpy(status.data(), cx.status, 4 * (size_t)n, hipMemcpyDeviceToHost) == hipSuccess;
 		std::vector<i32> again;
 		for (i32 r : todo) {
 			if (status[r] == GWF_ST_OVERFLOW) again.push_back(r);
-			else if (status[r] != GWF_ST_OK) {
-				set_err("gwf_ed_run: read %d ended with internal status %d", r, status[r]);
-				rc = GWF_ERR_STATE;
-				goto done;
-			}
+			else if (status[r] != GWF_ST_OK) { set_err("gwf_ed_run: read %d ended with internal status %d", r, status[r]); rc = GWF_ERR_STATE; }
 		}
 		if (!again.empty()) {
-			st.n_retry += (int64_t)again.size();
+			acc.st.n_retry += (int64_t)again.size();
 			scale *= 4;
-			if (caps_for(1, scale).a >= (1 << 26) && scale > 1 << 12) {
+			if (caps_for(1, scale).x >= (1 << 25) && scale > (1 << 12)) {
 				set_err("gwf_ed_run: %zu reads exceed the maximum workspace", again.size());
 				rc = GWF_ERR_CAPACITY;
-				goto done;
 			}
 		}
 		todo.swap(again);
 	}
-	RUNCHK(hipMemcpy(out, cx.ed, 4 * (size_t)n, hipMemcpyDeviceToHost));
-	RUNCHK(hipMemcpy(cnt.data(), cx.cnt, 8 * GWF_NCOUNT * (size_t)n, hipMemcpyDeviceToHost));
-	{
+	if (!ok) {
+		set_err("gwf_ed_run: %s failed: %s", acc.what, hipGetErrorString(acc.err));
+		rc = GWF_ERR_HIP;
+	}
+	if (rc == GWF_OK) {
+		if (hipMemcpy(out, cx.ed, 4 * (size_t)n, hipMemcpyDeviceToHost) != hipSuccess ||
+		    hipMemcpy(cnt.data(), cx.cnt, 8 * GWF_NCOUNT * (size_t)n, hipMemcpyDeviceToHost) != hipSuccess) {
+			set_err("gwf_ed_run: result copy failed");
+			rc = GWF_ERR_HIP;
+		}
+	}
+	if (rc == GWF_OK) {
 		int64_t tmin = INT64_MAX;
 		for (i32 i = 0; i < n; ++i) {
 			const int64_t *c = &cnt[GWF_NCOUNT * (size_t)i];
-			st.sum_steps += c[0];
-			st.sum_states += c[1];
-			st.sum_chars += c[2];
+			acc.st.sum_steps += c[0];
+			acc.st.sum_states += c[1];
+			acc.st.sum_chars += c[2];
+			acc.st.n_spill += c[5] >= 0;
 			tmin = std::min(tmin, c[3]);
 		}
 		const double ns_per_tick = cx.wall_khz > 0 ? 1e6 / cx.wall_khz : 10.0;
@@ -434,10 +571,8 @@ int gwf_ed_run_timed(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t v1, uint32_t ma
 			if (t_end_ns) t_end_ns[i] = (int64_t)((c[4] - tmin) * ns_per_tick);
 		}
 	}
-done:
-#undef RUNCHK
-	st.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-	if (stats) *stats = st;
+	acc.st.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+	if (stats) *stats = acc.st;
 	return rc;
 }
 

@@ -25,33 +25,39 @@ def _gpu_eds(gw, graph, reads, kernel=0):
         dg.free()
 
 
-def test_fixture_1000_reads(gpu):
-    _say("case: exposed_testing/1000_reads (golden EDs)")
+KERNELS = [0, 1]   # plain, persistent
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_fixture_1000_reads(gpu, kernel):
+    _say(f"case: exposed_testing/1000_reads (golden EDs), kernel {kernel}")
     g = gpu.Graph.from_gfa(os.path.join(GOLDEN, "graph_1000.gfa"))
     r = gpu.Reads.from_fasta(os.path.join(GOLDEN, "reads_1000.fa.gz"))
     want = np.array(read_ints(os.path.join(GOLDEN, "reads_1000.ed")))
-    got, st = _gpu_eds(gpu, g, r)
+    got, st = _gpu_eds(gpu, g, r, kernel)
     bad = np.nonzero(got != want)[0]
     assert bad.size == 0, f"{bad.size} mismatches, first {bad[:5]}: gpu {got[bad[:5]]} want {want[bad[:5]]}"
     assert st["sum_steps"] == int(want.sum()) + len(want)
 
 
-def test_fixture_grafo_test(gpu):
-    _say("case: exposed_testing/generated/grafo_test.gfa + sampled reads (golden EDs)")
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_fixture_grafo_test(gpu, kernel):
+    _say(f"case: exposed_testing/generated/grafo_test.gfa + sampled reads (golden EDs), kernel {kernel}")
     g = gpu.Graph.from_gfa(os.path.join(GOLDEN, "grafo_test.gfa"))
     r = gpu.Reads.from_fasta(os.path.join(GOLDEN, "grafo_reads.fa"))
     want = np.array(read_ints(os.path.join(GOLDEN, "grafo_reads.ed")))
-    got, _ = _gpu_eds(gpu, g, r)
+    got, _ = _gpu_eds(gpu, g, r, kernel)
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("c,n", [(1, 1000), (2, 240), (3, 300), (4, 12)])
-def test_configs_vs_oracle(gpu, orc, c, n):
-    _say(f"case: config {c}, first {n} reads")
+def test_configs_vs_oracle(gpu, orc, c, n, kernel):
+    _say(f"case: config {c}, first {n} reads, kernel {kernel}")
     g, r = gpu.make_workload(c, n)
     g.index()
     want, ost, _ = orc.batch(g, r)
-    got, st = _gpu_eds(gpu, g, r)
+    got, st = _gpu_eds(gpu, g, r, kernel)
     bad = np.nonzero(got != want)[0]
     assert bad.size == 0, f"config {c}: {bad.size} mismatches, first {bad[:5]}: gpu {got[bad[:5]]} oracle {want[bad[:5]]}"
     assert st["sum_steps"] == ost["steps"]
@@ -98,9 +104,10 @@ def _sample(rng, graph_seqs, arcs_of, ql, err):
     return "".join(out)
 
 
+@pytest.mark.parametrize("kernel", KERNELS)
 @pytest.mark.parametrize("seed,dag,ovl", [(1, True, False), (2, False, False), (3, True, True), (4, False, True)])
-def test_random_graphs_vs_oracle(gpu, orc, seed, dag, ovl):
-    _say(f"case: random graphs seed={seed} dag={dag} ovl={ovl}")
+def test_random_graphs_vs_oracle(gpu, orc, seed, dag, ovl, kernel):
+    _say(f"case: random graphs seed={seed} dag={dag} ovl={ovl} kernel {kernel}")
     rng = random.Random(seed)
     for it in range(25):
         g, seqs = _random_graph(gpu, rng, rng.randint(1, 40), dag, ovl)
@@ -113,12 +120,13 @@ def test_random_graphs_vs_oracle(gpu, orc, seed, dag, ovl):
         r = gpu.Reads.from_list(reads)
         g.index()
         want, _, _ = orc.batch(g, r)
-        got, _ = _gpu_eds(gpu, g, r)
+        got, _ = _gpu_eds(gpu, g, r, kernel)
         assert np.array_equal(got, want), f"iteration {it}: gpu {got.tolist()} oracle {want.tolist()}"
 
 
-def test_edge_cases(gpu, orc):
-    _say("case: edge cases")
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_edge_cases(gpu, orc, kernel):
+    _say(f"case: edge cases, kernel {kernel}")
     # short start vertex (< 32 bp) read as an exact prefix: the reference's
     # unsigned max_k skips this extension; exact semantics give 0
     g = gpu.Graph.from_lists(["ACGTACGTAC", "GGTTAACC", "TTTT"], [(0, 1), (1, 2)])
@@ -127,7 +135,7 @@ def test_edge_cases(gpu, orc):
     r = gpu.Reads.from_list([c for c, _ in cases])
     g.index()
     want, _, _ = orc.batch(g, r)
-    got, _ = _gpu_eds(gpu, g, r)
+    got, _ = _gpu_eds(gpu, g, r, kernel)
     assert np.array_equal(got, want)
     for (q, exp), x in zip(cases, got):
         if exp is not None:
@@ -135,7 +143,7 @@ def test_edge_cases(gpu, orc):
     # ED == ql through an all-insertion end (the reference's score loop would spin)
     g2 = gpu.Graph.from_lists(["A", "AAAA"], [(0, 1)])
     r2 = gpu.Reads.from_list(["TT", "TTTTTTTT"])
-    got2, _ = _gpu_eds(gpu, g2, r2)
+    got2, _ = _gpu_eds(gpu, g2, r2, kernel)
     assert got2.tolist() == [2, 8]
 
 
@@ -149,14 +157,40 @@ def test_per_read_start_vertex(gpu, orc):
     assert np.array_equal(got, want)
 
 
-def test_workspace_overflow_retry(gpu, orc, monkeypatch):
-    _say("case: forced tiny workspace -> overflow -> re-run")
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_workspace_overflow_retry(gpu, orc, monkeypatch, kernel):
+    _say(f"case: forced tiny workspace -> overflow -> re-run, kernel {kernel}")
     monkeypatch.setenv("GWF_DEBUG_CAP_A", "16")
+    monkeypatch.setenv("GWF_PERSIST_M", "64")
     g, r = gpu.make_workload(2, 30)
     g.index()
     want, _, _ = orc.batch(g, r)
-    got, st = _gpu_eds(gpu, g, r)
+    got, st = _gpu_eds(gpu, g, r, kernel)
     assert st["n_retry"] > 0
+    assert np.array_equal(got, want)
+
+
+def test_lds_spill_to_hbm(gpu, orc, monkeypatch):
+    _say("case: persistent kernel, small LDS capacity -> wavefronts spill to HBM mid-read")
+    monkeypatch.setenv("GWF_PERSIST_M", "128")
+    monkeypatch.setenv("GWF_PERSIST_NW", "1")
+    g, r = gpu.make_workload(2, 60)
+    g.index()
+    want, _, _ = orc.batch(g, r)
+    got, st = _gpu_eds(gpu, g, r, 1)
+    assert st["n_spill"] > 0 and st["n_retry"] == 0
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("nw,m", [(1, 256), (2, 512), (4, 2048)])
+def test_persistent_configurations(gpu, orc, monkeypatch, nw, m):
+    _say(f"case: persistent kernel nw={nw} lds m={m}")
+    monkeypatch.setenv("GWF_PERSIST_NW", str(nw))
+    monkeypatch.setenv("GWF_PERSIST_M", str(m))
+    g, r = gpu.make_workload(3, 200)
+    g.index()
+    want, _, _ = orc.batch(g, r)
+    got, st = _gpu_eds(gpu, g, r, 1)
     assert np.array_equal(got, want)
 
 
