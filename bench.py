@@ -67,11 +67,13 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=2, help="workload 1-4 (default 2)")
-    ap.add_argument("--kernel", choices=["plain", "persistent"], default="plain")
+    ap.add_argument("--kernel", choices=["plain", "persistent"], default="persistent")
     ap.add_argument("--reads", type=int, default=None, help="use only the first N reads")
     ap.add_argument("--cpu-sample", type=int, default=3000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--traffic", default=None, help="JSON with fetch/write bytes per launch from rocprofv3 --pmc")
+    ap.add_argument("--traffic", default=None,
+                    help="JSON with calibrated HBM bytes per step from rocprofv3 --pmc (default: "
+                         "profiles/traffic_<kernel>_<workload>.json when present)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (reads shard independently; "
@@ -113,12 +115,17 @@ def main():
     achieved = alg_bytes / kern_s / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-            "kernel": "gwf_plain_kernel", "kernel_ms_per_step": round(kern_s * 1e3, 3),
-            "launches_per_step": st["n_launch"], "alg_bytes_per_step": int(alg_bytes)}
-    if args.traffic and os.path.exists(args.traffic):
-        with open(args.traffic) as f:
+            "kernel": "gwf_plain_kernel" if args.kernel == "plain" else "gwf_persistent_kernel",
+            "kernel_ms_per_step": round(kern_s * 1e3, 3), "launches_per_step": st["n_launch"],
+            "alg_bytes_per_step": int(alg_bytes),
+            "definition": "achieved = (sum ql + 4 n + 32 W + 2 C) / summed HIP-event time of the step's launches; "
+                          "W, C = wavefront states and compared characters counted by the kernel"}
+    tpath = args.traffic or os.path.join(ROOT, "profiles", f"traffic_{args.kernel}_{cfg['name']}.json")
+    if os.path.exists(tpath) and (args.reads is None):
+        with open(tpath) as f:
             tr = json.load(f)
-        roof["traffic"] = tr.get("bytes_per_step")
+        roof["traffic"] = tr.get("hbm_bytes_per_step")
+        roof["traffic_source"] = os.path.relpath(tpath, ROOT)
     lat = st["t_end_ns"] / 1e6
     svc = (st["t_end_ns"] - st["t_start_ns"]) / 1e6
     out = {
