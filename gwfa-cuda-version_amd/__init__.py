@@ -72,6 +72,27 @@ class RunStats(C.Structure):
 _libs = {}
 
 
+class ServeOpts(C.Structure):
+    _fields_ = [("state_budget_bytes", C.c_int64), ("staging_bytes", C.c_int64), ("ring_slots", C.c_int32),
+                ("nw", C.c_int32), ("lds_states", C.c_int32), ("wgs", C.c_int32), ("park_ms", C.c_int32),
+                ("watchdog_ms", C.c_int32)]
+
+
+# gwf_serve_result_t as a numpy record (72 bytes, no padding holes)
+SERVE_RESULT = np.dtype([("id", "<i8"), ("ed", "<i4"), ("status", "<i4"), ("t_arrival_ns", "<i8"),
+                         ("t_admit_ns", "<i8"), ("t_done_ns", "<i8"), ("service_ns", "<i8"), ("steps", "<i8"),
+                         ("states", "<i8"), ("attempts", "<i4"), ("spilled", "<i4")])
+
+
+class ServeStats(C.Structure):
+    _fields_ = [(n, C.c_int64) for n in (
+        "submitted", "completed", "queued", "inflight", "readmitted", "rejected", "launches", "budget_bytes",
+        "budget_used", "budget_peak", "inflight_peak", "stalled", "bases_done")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 def _load(name):
     if name in _libs:
         return _libs[name]
@@ -131,6 +152,14 @@ def hip_lib():
         lib.gwf_ed_batch.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                      C.c_int32, C.c_uint32, C.c_int32, C.c_void_p, P(RunStats)]
         lib.gwf_last_error.restype = C.c_char_p
+        lib.gwf_serve_clock_ns.restype = C.c_int64
+        lib.gwf_serve_start.argtypes = [C.c_void_p, P(ServeOpts), P(C.c_void_p)]
+        lib.gwf_serve_submit.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.gwf_serve_poll.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32]
+        lib.gwf_serve_drain.argtypes = [C.c_void_p, C.c_int32]
+        lib.gwf_serve_stats.argtypes = [C.c_void_p, P(ServeStats)]
+        lib.gwf_serve_stop.argtypes = [C.c_void_p]
         lib._typed = True
     return lib
 
@@ -433,3 +462,81 @@ def gwf_ed(graph, q, v0=0, v1=-1):
     if r < -1:
         _check(r, "gwf_ed")
     return r
+
+
+class Server:
+    """Continuous-batching server (gwf_serve_*): the device graph stays
+    resident, requests are admitted by predicted state footprint under a fixed
+    device budget, and a resident device loop aligns them as they arrive."""
+
+    def __init__(self, dgraph, **opts):
+        self.lib = hip_lib()
+        self.dgraph = dgraph
+        o = ServeOpts(**opts)
+        self.ptr = C.c_void_p()
+        _check(self.lib.gwf_serve_start(dgraph.ptr, C.byref(o), C.byref(self.ptr)), "gwf_serve_start")
+
+    @staticmethod
+    def clock_ns():
+        return hip_lib().gwf_serve_clock_ns()
+
+    def submit(self, reads, release_ns=None):
+        """Queue every read of `reads` (a Reads); release_ns (int64 array,
+        server clock) delays each one's visibility.  Returns the ids."""
+        n = len(reads)
+        ids = np.empty(max(n, 1), np.int64)
+        rel = None
+        if release_ns is not None:
+            rel = np.ascontiguousarray(release_ns, dtype=np.int64)
+            assert len(rel) == n
+        _check(self.lib.gwf_serve_submit(self.ptr, n, reads.seqs.ctypes.data, reads.off.ctypes.data,
+                                         reads.ql.ctypes.data, reads.v0.ctypes.data,
+                                         rel.ctypes.data if rel is not None else None, ids.ctypes.data),
+               "gwf_serve_submit")
+        return ids[:n]
+
+    def poll(self, max_n=4096, wait_ms=0):
+        buf = np.zeros(max_n, SERVE_RESULT)
+        k = self.lib.gwf_serve_poll(self.ptr, buf.ctypes.data, max_n, wait_ms)
+        if k < 0:
+            _check(k, "gwf_serve_poll")
+        return buf[:k]
+
+    def drain(self, timeout_ms=-1):
+        _check(self.lib.gwf_serve_drain(self.ptr, timeout_ms), "gwf_serve_drain")
+
+    def collect(self, n, timeout_s=600.0):
+        """Poll until n results have been returned; results sorted by id."""
+        import time
+        out, got, t_end = [], 0, time.monotonic() + timeout_s
+        while got < n:
+            if time.monotonic() > t_end:
+                raise GwfError(f"Server.collect: {n - got} of {n} results missing after {timeout_s} s")
+            r = self.poll(8192, 50)
+            out.append(r)
+            got += len(r)
+        res = np.concatenate(out) if out else np.zeros(0, SERVE_RESULT)
+        return np.sort(res, order="id")
+
+    def stats(self):
+        st = ServeStats()
+        _check(self.lib.gwf_serve_stats(self.ptr, C.byref(st)), "gwf_serve_stats")
+        return st.as_dict()
+
+    def stop(self):
+        if self.ptr:
+            rc = self.lib.gwf_serve_stop(self.ptr)
+            self.ptr = None
+            _check(rc, "gwf_serve_stop")
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.stop()
+
+    def __del__(self):
+        try:
+            self.stop()
+        except Exception:
+            pass

@@ -77,12 +77,82 @@ hipError_t gwf_launch_plain(const DGraph &g, const DReads &r, const i32 *read_id
 
 // Persistent kernel: cfg.wgs workgroups pull reads order[head..n) through the
 // device counter *head until the queue is empty.
+// beat (may be null): one GwfBeat per workgroup in host-coherent memory.
+struct GwfBeat;
 hipError_t gwf_launch_persistent(const DGraph &g, const DReads &r, const i32 *order, i32 n, i32 *head,
                                  const PersistCfg &cfg, u64 *scratch, size_t ws_bytes, i32 *out_ed,
-                                 i32 *out_status, int64_t *out_counts, hipStream_t st);
+                                 i32 *out_status, int64_t *out_counts, GwfBeat *beat, hipStream_t st);
 
 // Largest dynamic LDS a workgroup may use on this device.
 size_t gwf_max_lds_bytes();
 
 // Resident persistent workgroups per CU for nw waves and lds_bytes of LDS.
 int gwf_persist_occupancy(int nw, size_t lds_bytes);
+
+// ------------------------------------------------------------ heartbeat
+// Progress record of one workgroup, written by the kernel into host-coherent
+// memory every GWF_BEAT_EVERY score steps (and when it picks up a read), so a
+// host thread can tell a slow read from a stuck one while the kernel runs.
+#define GWF_BEAT_EVERY 16
+struct GwfBeat {
+	u64 beat;     // increases with every record
+	i32 read;     // read index (batch) or request sequence number (serving); -1 idle
+	i32 step;     // its current score step
+	i32 n_a;      // its current wavefront size
+	i32 in_hbm;   // 1 once its state left LDS
+	u64 pad;
+};
+
+// ------------------------------------------------------------ serving
+// A request published to the device: the host writes the entry and the
+// read's bytes (host-coherent memory), then advances SrvCtl::word.  Entry
+// seq lives at ring[seq & mask] until a workgroup claims it; the host keeps
+// fewer than mask + 1 requests in flight, so it never overwrites an
+// unclaimed entry.  The result goes to res[res_slot], a slot the host owns
+// until it has read the result.
+struct SrvEntry {
+	u64 q_off;     // offset of the read's bytes in the staging arena (16-aligned,
+	               // zero-filled to the next 16 bytes and one more 16-byte word)
+	u64 d_off;     // offset of the request's region in the device pool: the
+	               // read's copy (kSeqPad past it), then its HBM workspace
+	u64 ws_off;    // offset of the HBM workspace inside the region
+	i32 ql, v0;
+	Caps hc;       // HBM workspace capacities
+	u32 res_slot;
+	i32 pad[2];
+};
+struct SrvResult {
+	i32 ed, status;
+	int64_t steps, states, chars, t0, t1, spill_step;
+	int64_t t_claim;  // device clock when a workgroup claimed the request
+	i32 wg;           // that workgroup
+	i32 polls;        // empty polls it made before the claim
+	u64 done;         // request sequence + 1 once the fields above are final
+};
+// word = tail (requests published) | stop << 63
+struct SrvCtl {
+	u64 word;
+	u64 pad[7];
+};
+#define GWF_SRV_STOP (1ull << 63)
+
+struct SrvArgs {
+	DGraph g;
+	const SrvEntry *ring;   // host-coherent, mask + 1 entries
+	SrvResult *res;         // host-coherent, mask + 1 results
+	const SrvCtl *ctl;      // host-coherent
+	u64 *claimed;           // device: requests claimed so far
+	const uint8_t *stage;   // host-coherent staging arena
+	uint8_t *pool;          // device pool
+	GwfBeat *beat;          // host-coherent, one per workgroup (may be null)
+	u64 mask;
+	Caps lc;                // LDS capacities
+	i32 idle_polls;         // a workgroup exits after this many empty polls
+};
+
+// Device server: cfg.wgs workgroups claim published requests until the stop
+// bit is set (and no request is left) or they have polled idle_polls times.
+hipError_t gwf_launch_serve(const SrvArgs &a, const PersistCfg &cfg, hipStream_t st);
+
+// Resident server workgroups per CU for nw waves and lds_bytes of LDS.
+int gwf_serve_occupancy(int nw, size_t lds_bytes);

@@ -769,10 +769,20 @@ struct ReadOut {
 	int64_t steps, states, chars, spill_step;
 };
 
-// Align one read: LDS-resident while it fits (USE_LDS), then in HBM.
+// Heartbeat record (relaxed system-scope stores: the host only reads it).
+__device__ __forceinline__ void beat_put(GwfBeat *bt, i32 rid, i32 s, i32 n_a, i32 hbm) {
+	__hip_atomic_store(&bt->read, rid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+	__hip_atomic_store(&bt->step, s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+	__hip_atomic_store(&bt->n_a, n_a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+	__hip_atomic_store(&bt->in_hbm, hbm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+	__hip_atomic_store(&bt->beat, (u64)wall_clock64(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Align one read: LDS-resident while it fits (USE_LDS), then in HBM.  beat
+// (may be null) receives a progress record every GWF_BEAT_EVERY steps.
 template <int NW, bool USE_LDS>
 __device__ __forceinline__ ReadOut align_read(const DGraph &g, const uint8_t *q, i32 ql, u32 v0, const Bufs &lds, const Bufs &hbm,
-                              const Blk<NW> &b, u64 *vis) {
+                              const Blk<NW> &b, u64 *vis, GwfBeat *beat, i32 rid) {
 	ReadOut o;
 	o.ed = -1, o.status = GWF_ST_OK, o.steps = 0, o.states = 0, o.chars = 0, o.spill_step = -1;
 	i32 n_a = 1, n_h = 0, hs = 0;
@@ -784,6 +794,7 @@ __device__ __forceinline__ ReadOut align_read(const DGraph &g, const uint8_t *q,
 		o.steps += 1;
 		o.states += n_a;
 		if (s > ql + 1) { o.status = GWF_ST_LIMIT; break; }
+		if (beat && (s % GWF_BEAT_EVERY) == 0 && b.tid() == 0) beat_put(beat, rid, s, n_a, !in_lds);
 		int rc = STEP_SPILL;
 		if constexpr (USE_LDS) {
 			if (in_lds) {
@@ -847,8 +858,22 @@ __global__ __launch_bounds__(WAVE) void gwf_plain_kernel(DGraph g, DReads rd, co
 	w.sort_lds = lds_sort;
 	w.sort_cap = SORT_CHUNK;
 	const u64 t0 = wall_clock64();
-	const ReadOut o = align_read<1, false>(g, rd.seq + rd.off[r], rd.ql[r], (u32)rd.v0[r], w, w, b, vis);
+	const ReadOut o = align_read<1, false>(g, rd.seq + rd.off[r], rd.ql[r], (u32)rd.v0[r], w, w, b, vis, nullptr, r);
 	write_out<1>(o, r, t0, out_ed, out_status, out_counts, b);
+}
+
+__device__ __forceinline__ u64 uni64(u64 x) {
+	return (u64)(u32)__builtin_amdgcn_readfirstlane((i32)(u32)x) |
+	       (u64)(u32)__builtin_amdgcn_readfirstlane((i32)(u32)(x >> 32)) << 32;
+}
+
+// Largest power-of-two run that the LDS arena of capacities lc can sort once
+// a read has spilled (the arena is free then).
+__device__ __forceinline__ i32 arena_sort_cap(const Caps &lc) {
+	const size_t le = lds_elems(lc);
+	i32 sc = 2;
+	while ((size_t)sc * 2 <= le && sc < (1 << 14)) sc <<= 1;
+	return sc;
 }
 
 // Persistent: resident workgroups pull reads from a queue; each read is
@@ -857,7 +882,7 @@ template <int NW>
 __global__ __launch_bounds__(WAVE *NW) void gwf_persistent_kernel(DGraph g, DReads rd, const i32 *order, i32 n,
                                                                  i32 *head, Caps lc, Caps hc, u64 *scratch,
                                                                  u64 ws_elems_per_wg, i32 *out_ed, i32 *out_status,
-                                                                 int64_t *out_counts) {
+                                                                 int64_t *out_counts, GwfBeat *beats) {
 	extern __shared__ u64 arena[];
 	__shared__ u64 vis[VIS_SLOTS];
 	__shared__ i32 red[NW];
@@ -867,12 +892,8 @@ __global__ __launch_bounds__(WAVE *NW) void gwf_persistent_kernel(DGraph g, DRea
 	const Bufs lb = carve(arena, lc, false);
 	Bufs hb = carve(scratch + (size_t)blockIdx.x * ws_elems_per_wg, hc, true);
 	hb.sort_lds = arena;  // the LDS arena is free once a read has spilled
-	{
-		const size_t le = lds_elems(lc);
-		i32 sc = 2;
-		while ((size_t)sc * 2 <= le && sc < (1 << 14)) sc <<= 1;
-		hb.sort_cap = sc;
-	}
+	hb.sort_cap = arena_sort_cap(lc);
+	GwfBeat *beat = beats ? beats + blockIdx.x : nullptr;
 	for (;;) {
 		if (b.tid() == 0) next_q = atomicAdd(head, 1);
 		b.sync();
@@ -881,8 +902,142 @@ __global__ __launch_bounds__(WAVE *NW) void gwf_persistent_kernel(DGraph g, DRea
 		if (qi >= n) break;
 		const i32 r = order[qi];
 		const u64 t0 = wall_clock64();
-		const ReadOut o = align_read<NW, true>(g, rd.seq + rd.off[r], rd.ql[r], (u32)rd.v0[r], lb, hb, b, vis);
+		const ReadOut o = align_read<NW, true>(g, rd.seq + rd.off[r], rd.ql[r], (u32)rd.v0[r], lb, hb, b, vis, beat, r);
 		write_out<NW>(o, r, t0, out_ed, out_status, out_counts, b);
+		b.sync();
+	}
+	if (beat && b.tid() == 0) beat_put(beat, -1, 0, 0, 0);
+}
+
+// Device server: resident workgroups claim requests published by the host
+// in a ring of host-coherent memory, copy the read into the request's
+// device region, align it (LDS first, then the region's HBM workspace) and
+// publish the result with a system-scope release, so the host sees every
+// read as soon as it is done while the kernel keeps running.  Requests that
+// arrive while others run are claimed by whichever workgroup frees up first.
+//
+// Host memory is read with relaxed system-scope atomic loads (coherent
+// without cache invalidation), so idle workgroups polling the ring do not
+// disturb the caches of busy ones; the request's device region may have been
+// used by a workgroup on another XCD, hence the agent-scope acquire before
+// using it and the system-scope release that ends every request.
+__device__ __forceinline__ u64 host_ld(const u64 *p) {
+	return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+template <int NW>
+__global__ __launch_bounds__(WAVE *NW) void gwf_serve_kernel(SrvArgs a) {
+	extern __shared__ u64 arena[];
+	__shared__ u64 vis[VIS_SLOTS];
+	__shared__ i32 red[NW];
+	__shared__ u64 redu[NW];
+	__shared__ int64_t s_seq, s_claim;
+	__shared__ i32 s_polls;
+	__shared__ SrvEntry s_e;
+	constexpr i32 NT = Blk<NW>::NT;
+	static_assert(sizeof(SrvEntry) == 64, "SrvEntry is eight u64 words");
+	Blk<NW> b{red, redu};
+	const Bufs lb = carve(arena, a.lc, false);
+	const i32 sort_cap = arena_sort_cap(a.lc);
+	GwfBeat *beat = a.beat ? a.beat + blockIdx.x : nullptr;
+	i32 idle = 0;
+	u64 claimed_seen = 0;  // thread 0: a lower bound of the claim counter
+	for (;;) {
+		if (b.tid() == 0) {
+			// Idle polling reads only the host's tail word, with exponential
+			// back-off (1 to 32 sleeps of 127 x 64 cycles, ~4 to ~125 us): a
+			// thousand idle workgroups hammering one device counter or the
+			// host link would slow the busy ones down.  The claim counter is
+			// only touched when the tail passes the last count seen (c may be
+			// stale-low; a failed compare-exchange refreshes it).
+			int64_t got = -1;
+			u64 c = claimed_seen;
+			for (;;) {
+				const u64 w = host_ld(&a.ctl->word);
+				const u64 tail = w & ~GWF_SRV_STOP;
+				if (c < tail) {
+					if (__hip_atomic_compare_exchange_strong(a.claimed, &c, c + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+					                                         __HIP_MEMORY_SCOPE_AGENT)) {
+						got = (int64_t)c;
+						c += 1;
+						break;
+					}
+					continue;  // c now holds the current count
+				}
+				if ((w & GWF_SRV_STOP) || ++idle > a.idle_polls) break;
+				const i32 naps = idle < 40 ? 1 << (idle >> 3) : 32;
+				for (i32 z = 0; z < naps; ++z) __builtin_amdgcn_s_sleep(127);
+			}
+			claimed_seen = c;
+			if (got >= 0) {
+				s_claim = (int64_t)wall_clock64();
+				s_polls = idle;
+				const u64 *src = (const u64 *)&a.ring[(u64)got & a.mask];
+				u64 *dst = (u64 *)&s_e;
+#pragma unroll
+				for (int t = 0; t < 8; ++t) dst[t] = host_ld(src + t);
+				idle = 0;
+				if (beat) beat_put(beat, (i32)got, 0, 1, 0);
+			}
+			s_seq = got;
+		}
+		b.sync();
+		const int64_t seq = s_seq;
+		if (seq < 0) break;
+		// the entry is uniform: keep it in scalar registers
+		SrvEntry e;
+		e.q_off = uni64(s_e.q_off);
+		e.d_off = uni64(s_e.d_off);
+		e.ws_off = uni64(s_e.ws_off);
+		e.ql = __builtin_amdgcn_readfirstlane(s_e.ql);
+		e.v0 = __builtin_amdgcn_readfirstlane(s_e.v0);
+		e.hc.m = __builtin_amdgcn_readfirstlane(s_e.hc.m);
+		e.hc.x = __builtin_amdgcn_readfirstlane(s_e.hc.x);
+		e.hc.f = __builtin_amdgcn_readfirstlane(s_e.hc.f);
+		e.hc.t = __builtin_amdgcn_readfirstlane(s_e.hc.t);
+		e.hc.h = __builtin_amdgcn_readfirstlane(s_e.hc.h);
+		e.res_slot = (u32)__builtin_amdgcn_readfirstlane((i32)s_e.res_slot);
+		uint8_t *dq = a.pool + e.d_off;
+		{
+			const u64 *src = (const u64 *)(a.stage + e.q_off);
+			const i32 n8 = ((e.ql + 15) / 16 + 1) * 2;
+			for (i32 t = (i32)b.tid(); t < n8; t += NT) ((u64 *)dq)[t] = host_ld(src + t);
+		}
+		b.sync();
+		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+		Bufs hb = carve((u64 *)(a.pool + e.d_off + e.ws_off), e.hc, true);
+		hb.sort_lds = arena;
+		hb.sort_cap = sort_cap;
+		const u64 t0 = wall_clock64();
+		const ReadOut o = align_read<NW, true>(a.g, dq, e.ql, (u32)e.v0, lb, hb, b, vis, beat, (i32)seq);
+		int64_t c = o.chars;
+#pragma unroll
+		for (int s = 32; s > 0; s >>= 1) c += __shfl_xor(c, s);
+		if constexpr (NW > 1) {
+			if (lane_id() == 0) b.redu[b.wid()] = (u64)c;
+			b.sync();
+			c = 0;
+			for (int w = 0; w < NW; ++w) c += (int64_t)b.redu[w];
+		}
+		b.sync();  // every wave is done with the region
+		if (b.tid() == 0) {
+			SrvResult *r = &a.res[e.res_slot];
+			r->ed = o.ed;
+			r->status = o.status;
+			r->steps = o.steps;
+			r->states = o.states;
+			r->chars = c;
+			r->t0 = (int64_t)t0;
+			r->t1 = (int64_t)wall_clock64();
+			r->spill_step = o.spill_step;
+			r->t_claim = s_claim;
+			r->wg = (i32)blockIdx.x;
+			r->polls = s_polls;
+			// release: the result fields and the workgroup's writes to the
+			// request's region are visible before the host sees `done`
+			__hip_atomic_store(&r->done, (u64)seq + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+			if (beat) beat_put(beat, -1, 0, 0, 0);
+		}
 		b.sync();
 	}
 }
@@ -899,7 +1054,7 @@ hipError_t gwf_launch_plain(const DGraph &g, const DReads &r, const i32 *read_id
 template <int NW>
 static hipError_t launch_persist(const DGraph &g, const DReads &r, const i32 *order, i32 n, i32 *head,
                                  const PersistCfg &cfg, u64 *scratch, size_t ws_bytes, i32 *out_ed, i32 *out_status,
-                                 int64_t *out_counts, hipStream_t st) {
+                                 int64_t *out_counts, GwfBeat *beat, hipStream_t st) {
 	static bool attr_set = false;
 	if (!attr_set) {
 		hipError_t e = hipFuncSetAttribute((const void *)gwf_persistent_kernel<NW>,
@@ -908,18 +1063,40 @@ static hipError_t launch_persist(const DGraph &g, const DReads &r, const i32 *or
 		attr_set = true;
 	}
 	hipLaunchKernelGGL(gwf_persistent_kernel<NW>, dim3(cfg.wgs), dim3(WAVE * NW), cfg.lds_bytes, st, g, r, order, n,
-	                   head, cfg.lds, cfg.hbm, scratch, (u64)(ws_bytes / 8), out_ed, out_status, out_counts);
+	                   head, cfg.lds, cfg.hbm, scratch, (u64)(ws_bytes / 8), out_ed, out_status, out_counts, beat);
 	return hipGetLastError();
+}
+
+template <int NW>
+static hipError_t launch_serve(const SrvArgs &a, const PersistCfg &cfg, hipStream_t st) {
+	static bool attr_set = false;
+	if (!attr_set) {
+		hipError_t e = hipFuncSetAttribute((const void *)gwf_serve_kernel<NW>,
+		                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)gwf_max_lds_bytes());
+		if (e != hipSuccess) return e;
+		attr_set = true;
+	}
+	hipLaunchKernelGGL(gwf_serve_kernel<NW>, dim3(cfg.wgs), dim3(WAVE * NW), cfg.lds_bytes, st, a);
+	return hipGetLastError();
+}
+
+hipError_t gwf_launch_serve(const SrvArgs &a, const PersistCfg &cfg, hipStream_t st) {
+	switch (cfg.nw) {
+	case 1: return launch_serve<1>(a, cfg, st);
+	case 2: return launch_serve<2>(a, cfg, st);
+	case 4: return launch_serve<4>(a, cfg, st);
+	default: return hipErrorInvalidValue;
+	}
 }
 
 hipError_t gwf_launch_persistent(const DGraph &g, const DReads &r, const i32 *order, i32 n, i32 *head,
                                  const PersistCfg &cfg, u64 *scratch, size_t ws_bytes, i32 *out_ed, i32 *out_status,
-                                 int64_t *out_counts, hipStream_t st) {
+                                 int64_t *out_counts, GwfBeat *beat, hipStream_t st) {
 	if (n <= 0) return hipSuccess;
 	switch (cfg.nw) {
-	case 1: return launch_persist<1>(g, r, order, n, head, cfg, scratch, ws_bytes, out_ed, out_status, out_counts, st);
-	case 2: return launch_persist<2>(g, r, order, n, head, cfg, scratch, ws_bytes, out_ed, out_status, out_counts, st);
-	case 4: return launch_persist<4>(g, r, order, n, head, cfg, scratch, ws_bytes, out_ed, out_status, out_counts, st);
+	case 1: return launch_persist<1>(g, r, order, n, head, cfg, scratch, ws_bytes, out_ed, out_status, out_counts, beat, st);
+	case 2: return launch_persist<2>(g, r, order, n, head, cfg, scratch, ws_bytes, out_ed, out_status, out_counts, beat, st);
+	case 4: return launch_persist<4>(g, r, order, n, head, cfg, scratch, ws_bytes, out_ed, out_status, out_counts, beat, st);
 	default: return hipErrorInvalidValue;
 	}
 }
@@ -941,6 +1118,25 @@ int gwf_persist_occupancy(int nw, size_t lds_bytes) {
 	case 1: return occ<1>(lds_bytes);
 	case 2: return occ<2>(lds_bytes);
 	case 4: return occ<4>(lds_bytes);
+	default: return 1;
+	}
+}
+
+template <int NW>
+static int occ_serve(size_t lds_bytes) {
+	int nb = 0;
+	(void)hipFuncSetAttribute((const void *)gwf_serve_kernel<NW>, hipFuncAttributeMaxDynamicSharedMemorySize,
+	                          (int)gwf_max_lds_bytes());
+	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, gwf_serve_kernel<NW>, WAVE * NW, lds_bytes) != hipSuccess)
+		return 1;
+	return nb;
+}
+
+int gwf_serve_occupancy(int nw, size_t lds_bytes) {
+	switch (nw) {
+	case 1: return occ_serve<1>(lds_bytes);
+	case 2: return occ_serve<2>(lds_bytes);
+	case 4: return occ_serve<4>(lds_bytes);
 	default: return 1;
 	}
 }

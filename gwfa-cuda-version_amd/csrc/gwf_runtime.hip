@@ -9,12 +9,14 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <thread>
 #include <string>
 #include <unordered_map>
 #include <vector>
 
 #include "gwf_dev.h"
-#include "../../include/gwfa.h"
+#include "gwf_policy.h"
+#include "gwfa.h"
 
 namespace {
 
@@ -38,7 +40,6 @@ void set_err(const char *fmt, ...) {
 		}                                                                          \
 	} while (0)
 
-constexpr size_t kSeqPad = 2048;  // bytes readable past the end of packed sequences
 constexpr i32 kProbeLen = 256;    // prefix aligned to estimate each read's error rate
 
 int device_ready() {
@@ -51,12 +52,6 @@ int device_ready() {
 }
 
 }  // namespace
-
-struct gwf_dgraph_s {
-	DGraph g;
-	void *mem = nullptr;
-	u32 n_vtx = 0;
-};
 
 struct gwf_dreads_s {
 	DReads r;
@@ -82,6 +77,8 @@ struct RunCtx {
 	size_t n_cap = 0;
 	u64 *scratch = nullptr;
 	size_t scratch_bytes = 0;
+	GwfBeat *beat = nullptr;  // host-coherent heartbeat records
+	size_t beat_n = 0;
 };
 RunCtx g_run;
 
@@ -183,6 +180,18 @@ std::mutex g_cache_mu;
 std::unordered_map<const gwf_graph_t *, gwf_dgraph_t *> g_cache;
 
 }  // namespace
+
+int64_t gwf_env_int(const char *name, int64_t dflt) { return env_int(name, dflt); }
+Caps gwf_caps_for(i32 ql, int scale) { return caps_for(ql, scale); }
+Caps gwf_lds_caps(i32 m) { return lds_caps(m); }
+void gwf_set_error(const char *fmt, ...) {
+	char buf[512];
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(buf, sizeof(buf), fmt, ap);
+	va_end(ap);
+	g_err = buf;
+}
 
 extern "C" {
 
@@ -391,10 +400,50 @@ struct RunAcc {  // statistics of one gwf_ed_run call
 		}                                                   \
 	} while (0)
 
-bool timed_launch(RunCtx &cx, RunAcc &acc, const std::function<hipError_t()> &launch) {
+// Heartbeat (GWF_HEARTBEAT_MS > 0): while a launch runs, print one line per
+// period to stderr from the workgroups' progress records (main.cu:48-149's
+// progress slots, read from host-coherent memory instead of a debug build).
+struct Heartbeat {
+	GwfBeat *beat = nullptr;
+	i32 n = 0;
+	int period_ms = 0;
+	const char *what = "";
+};
+
+void heartbeat_wait(RunCtx &cx, const Heartbeat &hb) {
+	const auto t0 = std::chrono::steady_clock::now();
+	auto last = t0;
+	std::vector<u64> prev(hb.n, 0);
+	std::vector<int> still(hb.n, 0);
+	while (hipEventQuery(cx.e1) == hipErrorNotReady) {
+		const auto now = std::chrono::steady_clock::now();
+		if (std::chrono::duration<double, std::milli>(now - last).count() >= hb.period_ms) {
+			i32 busy = 0, hbm = 0, stuck = 0, max_step = 0, max_na = 0;
+			for (i32 w = 0; w < hb.n; ++w) {
+				const volatile GwfBeat &b = hb.beat[w];
+				if (b.read < 0) continue;
+				++busy;
+				hbm += b.in_hbm != 0;
+				max_step = std::max(max_step, (i32)b.step);
+				max_na = std::max(max_na, (i32)b.n_a);
+				still[w] = b.beat == prev[w] ? still[w] + 1 : 0;
+				stuck += still[w] >= 5;
+				prev[w] = b.beat;
+			}
+			fprintf(stderr, "[gwf heartbeat] %s t=%.1fs workgroups busy=%d/%d in_hbm=%d max_step=%d max_n_a=%d silent>=5=%d\n",
+			        hb.what, std::chrono::duration<double>(now - t0).count(), busy, hb.n, hbm, max_step, max_na, stuck);
+			fflush(stderr);
+			last = now;
+		}
+		std::this_thread::sleep_for(std::chrono::milliseconds(1));
+	}
+}
+
+bool timed_launch(RunCtx &cx, RunAcc &acc, const std::function<hipError_t()> &launch, const Heartbeat *hb = nullptr) {
 	ACC(hipEventRecord(cx.e0, cx.s));
 	ACC(launch());
 	ACC(hipEventRecord(cx.e1, cx.s));
+	if (hb && hb->beat) heartbeat_wait(cx, *hb);
 	ACC(hipEventSynchronize(cx.e1));
 	float ms = 0;
 	ACC(hipEventElapsedTime(&ms, cx.e0, cx.e1));
@@ -438,10 +487,25 @@ bool run_persistent(RunCtx &cx, RunAcc &acc, const gwf_dgraph_t *dg, const DRead
 	ACC(grow(&cx.scratch, &cx.scratch_bytes, ws * (size_t)cfg.wgs));
 	ACC(hipMemcpyAsync(cx.idx, order.data(), 4 * (size_t)n, hipMemcpyHostToDevice, cx.s));
 	ACC(hipMemsetAsync(cx.head, 0, sizeof(i32), cx.s));
+	Heartbeat hb;
+	hb.period_ms = (int)env_int("GWF_HEARTBEAT_MS", 0);
+	if (hb.period_ms > 0) {
+		if ((size_t)cfg.wgs > cx.beat_n) {
+			if (cx.beat) (void)hipHostFree(cx.beat);
+			cx.beat = nullptr;
+			cx.beat_n = 0;
+			ACC(hipHostMalloc((void **)&cx.beat, sizeof(GwfBeat) * cfg.wgs, hipHostMallocCoherent | hipHostMallocMapped));
+			cx.beat_n = cfg.wgs;
+		}
+		for (i32 w = 0; w < cfg.wgs; ++w) cx.beat[w].read = -1, cx.beat[w].beat = 0;
+		hb.beat = cx.beat;
+		hb.n = cfg.wgs;
+		hb.what = "persistent";
+	}
 	return timed_launch(cx, acc, [&] {
 		return gwf_launch_persistent(dg->g, r, cx.idx, n, cx.head, cfg, cx.scratch, ws, cx.ed, cx.status, cx.cnt,
-		                             cx.s);
-	});
+		                             hb.beat, cx.s);
+	}, &hb);
 }
 
 #undef ACC

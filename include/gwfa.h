@@ -192,6 +192,82 @@ int gwf_ed_batch(gwf_dgraph_t *dg, int32_t n, const char *const *q, const int32_
 /* Text of the last error seen by this thread ("" if none). */
 const char *gwf_last_error(void);
 
+/* ---- additions: serving (continuous batching) ----------------------------- */
+/* No counterpart in the reference, whose driver aligns one fixed batch
+ * (main.cu:454-750).  A server keeps the device graph resident and runs a
+ * device loop that claims requests as they are published: a request that
+ * arrives while others are aligning is admitted as soon as a ring slot and
+ * state budget are free, and its alignment overlaps theirs on the GPU.  Each
+ * request reserves its predicted state footprint (its read's device copy plus
+ * the HBM workspace its length calls for) from a fixed device budget until
+ * its result is returned.  A request whose workspace overflows is re-admitted
+ * with a 4x larger one; results are bit-identical to gwf_ed_run's. */
+
+typedef struct gwf_serve_s gwf_serve_t;
+
+typedef struct {
+	int64_t state_budget_bytes; /* device bytes for admitted requests (0: 1 GiB) */
+	int64_t staging_bytes;      /* pinned host bytes for admitted reads (0: 64 MiB) */
+	int32_t ring_slots;         /* most requests on the device at once, power of two (0: 4096) */
+	int32_t nw;                 /* waves per read: 1, 2 or 4 (0: 4) */
+	int32_t lds_states;         /* LDS wavefront capacity per read (0: 1024) */
+	int32_t wgs;                /* resident workgroups (0: occupancy x CUs) */
+	int32_t park_ms;            /* idle time after which the device loop is parked (0: 20) */
+	int32_t watchdog_ms;        /* heartbeat silence that counts a busy workgroup as stalled (0: 10000) */
+} gwf_serve_opts_t;
+
+typedef struct {
+	int64_t id;            /* as returned by gwf_serve_submit */
+	int32_t ed;            /* edit distance (GWF_ED_NONE: no end) */
+	int32_t status;        /* GWF_OK, GWF_ERR_BUDGET or GWF_ERR_CAPACITY */
+	int64_t t_arrival_ns;  /* gwf_serve_clock_ns() at which the request became visible */
+	int64_t t_admit_ns;    /* published to the device (last attempt) */
+	int64_t t_done_ns;     /* result seen by the server thread */
+	int64_t service_ns;    /* device time from claim to result (last attempt) */
+	int64_t steps;         /* score steps (ED + 1) */
+	int64_t states;        /* wavefront states processed */
+	int32_t attempts;      /* 1 + re-admissions after a workspace overflow */
+	int32_t spilled;       /* 1 if the state left LDS for HBM */
+} gwf_serve_result_t;
+
+typedef struct {
+	int64_t submitted, completed, queued, inflight;
+	int64_t readmitted;    /* re-admissions after workspace overflow */
+	int64_t rejected;      /* requests that can never fit the budget */
+	int64_t launches;      /* device-loop launches (relaunched after parking) */
+	int64_t budget_bytes, budget_used, budget_peak;
+	int64_t inflight_peak;
+	int64_t stalled;       /* workgroups whose heartbeat went silent past watchdog_ms */
+	int64_t bases_done;
+} gwf_serve_stats_t;
+
+/* Steady clock (ns) used for every t_* field and for release times. */
+int64_t gwf_serve_clock_ns(void);
+
+/* Start a server on the current device for dg (which must outlive it). */
+int gwf_serve_start(gwf_dgraph_t *dg, const gwf_serve_opts_t *opts, gwf_serve_t **out);
+
+/* Queue n requests: read i is seqs + off[i], ql[i] bytes, start vertex v0[i]
+ * (v0 NULL: 0).  t_release_ns[i] (NULL: now) is the gwf_serve_clock_ns() time
+ * from which request i is visible to admission (a load generator's arrival
+ * time).  ids[i] (may be NULL) receives each request's id.  Copies the bytes. */
+int gwf_serve_submit(gwf_serve_t *s, int32_t n, const char *seqs, const int64_t *off, const int32_t *ql,
+                     const int32_t *v0, const int64_t *t_release_ns, int64_t *ids);
+
+/* Move up to max finished results into out, waiting up to wait_ms for the
+ * first one.  Returns the number moved (>= 0) or a GWF_ERR_* code. */
+int gwf_serve_poll(gwf_serve_t *s, gwf_serve_result_t *out, int32_t max, int32_t wait_ms);
+
+/* Wait until every submitted request has a result (timeout_ms < 0: no limit);
+ * GWF_ERR_STATE on timeout. */
+int gwf_serve_drain(gwf_serve_t *s, int32_t timeout_ms);
+
+int gwf_serve_stats(gwf_serve_t *s, gwf_serve_stats_t *out);
+
+/* Drop requests not yet admitted, wait for those on the device, stop the
+ * device loop and free the server. */
+int gwf_serve_stop(gwf_serve_t *s);
+
 #ifdef __cplusplus
 }
 #endif

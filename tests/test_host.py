@@ -124,3 +124,28 @@ def test_host_entry_points_fail_cleanly_without_device(gw):
     assert gw.gwf_ed.__doc__
     lib = gw.hip_lib()
     assert lib.gwf_ed(g.ptr, 4, b"ACGT", 0, -1, 0, None, None) == -5   # GWF_ERR_NODEV
+
+
+def test_serving_type_layout(tmp_path, gw):
+    """The serving structs have the sizes the ctypes/numpy mirrors assume."""
+    import ctypes
+    src = tmp_path / "s.c"
+    src.write_text('#include <stdio.h>\n#include "gwfa.h"\n'
+                   'int main(void){printf("%zu %zu %zu\\n", sizeof(gwf_serve_opts_t), sizeof(gwf_serve_result_t), '
+                   'sizeof(gwf_serve_stats_t));return 0;}\n')
+    exe = tmp_path / "s"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert out == [ctypes.sizeof(gw.ServeOpts), gw.SERVE_RESULT.itemsize, ctypes.sizeof(gw.ServeStats)]
+
+
+def test_serve_start_fails_cleanly_without_device(gw):
+    from conftest import gpu_present
+    if gpu_present():
+        pytest.skip("device present")
+    import ctypes
+    lib = gw.hip_lib()
+    srv = ctypes.c_void_p()
+    assert lib.gwf_serve_start(ctypes.c_void_p(1), None, ctypes.byref(srv)) == -5   # GWF_ERR_NODEV
+    assert lib.gwf_serve_start(None, None, ctypes.byref(srv)) == -2                 # GWF_ERR_ARG
+    assert lib.gwf_serve_stop(None) == 0
