@@ -146,6 +146,10 @@ def main():
     ap.add_argument("--reads", type=int, default=None, help="use only the first N reads")
     ap.add_argument("--cpu-sample", type=int, default=3000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tune", default=None, help="persistent kernel: load this bin table (no autotuning)")
+    ap.add_argument("--autotune", type=int, default=-1,
+                    help="persistent kernel: autotune the bin table on the first N reads before timing, untimed "
+                         "(-1: all reads; 0: use the built-in table)")
     ap.add_argument("--serve", action="store_true", help="serving sweep (Poisson arrivals) instead of batch throughput")
     ap.add_argument("--rates", default="0.25,0.5,0.75,0.9,1.0", help="--serve: loads as fractions of saturation")
     ap.add_argument("--serve-seconds", type=float, default=1.5, help="--serve: arrival window per load")
@@ -170,6 +174,23 @@ def main():
     log(f"workload {cfg['name']}: {graph.n_vtx} vertices, {graph.n_arc} arcs, {len(reads)} reads, "
         f"{reads.bases()} bases ({time.perf_counter() - t:.1f} s to generate)")
     dg, dr = gw.DeviceGraph(graph), gw.DeviceReads(reads)
+    tuning = None
+    if kernel == gw.KERNEL_PERSISTENT and args.tune:
+        gw.tune_load(args.tune)
+        tuning = gw.tune_get()
+        args.autotune = 0
+    elif kernel == gw.KERNEL_PERSISTENT and args.autotune != 0:
+        t = time.perf_counter()
+        if args.autotune < 0 or args.autotune >= len(reads):
+            tuning, report = gw.autotune(dg, dr)
+        else:
+            dsample = gw.DeviceReads(reads.subset(np.arange(args.autotune)))
+            tuning, report = gw.autotune(dg, dsample)
+            dsample.free()
+        log(report.rstrip())
+        log(f"autotune: {time.perf_counter() - t:.1f} s (untimed)")
+    elif kernel == gw.KERNEL_PERSISTENT:
+        tuning = gw.tune_get()
 
     for i in range(args.warmup):
         eds0, st = gw.ed_run(dg, dr, kernel=kernel)
@@ -215,7 +236,9 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "u8+int32", "data": "synthetic",
         "config": {"workload": cfg["name"], "n_reads": len(reads), "bases": bases, "kernel": args.kernel,
                    "graph_vertices": graph.n_vtx, "graph_seed": cfg["graph"]["seed"],
-                   "read_seed": cfg["reads"]["seed"], "parallelism": "reads across workgroups, 1 GPU"},
+                   "read_seed": cfg["reads"]["seed"], "parallelism": "reads across workgroups, 1 GPU",
+                   "bins": tuning, "autotuned": bool(tuning is not None and args.autotune != 0),
+                   "graph_replay": kernel == gw.KERNEL_PERSISTENT and os.environ.get("GWF_GRAPH", "1") != "0"},
         "reads_per_s": round(len(reads) * args.steps / elapsed, 1),
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
                        "service_p50": round(float(np.percentile(svc, 50)), 3),

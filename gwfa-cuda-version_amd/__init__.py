@@ -72,6 +72,26 @@ class RunStats(C.Structure):
 _libs = {}
 
 
+class TuneBin(C.Structure):
+    _fields_ = [("est_max", C.c_int32), ("nw", C.c_int32), ("lds_states", C.c_int32), ("merge_runs", C.c_int32)]
+
+
+class Tune(C.Structure):
+    _fields_ = [("n_bins", C.c_int32), ("bin", TuneBin * 8)]
+
+    def as_list(self):
+        return [dict(est_max=b.est_max, nw=b.nw, lds_states=b.lds_states, merge_runs=b.merge_runs)
+                for b in self.bin[:self.n_bins]]
+
+    @classmethod
+    def from_list(cls, bins):
+        t = cls()
+        t.n_bins = len(bins)
+        for i, b in enumerate(bins):
+            t.bin[i] = TuneBin(b["est_max"], b["nw"], b["lds_states"], b["merge_runs"])
+        return t
+
+
 class ServeOpts(C.Structure):
     _fields_ = [("state_budget_bytes", C.c_int64), ("staging_bytes", C.c_int64), ("ring_slots", C.c_int32),
                 ("nw", C.c_int32), ("lds_states", C.c_int32), ("wgs", C.c_int32), ("park_ms", C.c_int32),
@@ -153,6 +173,11 @@ def hip_lib():
                                      C.c_int32, C.c_uint32, C.c_int32, C.c_void_p, P(RunStats)]
         lib.gwf_last_error.restype = C.c_char_p
         lib.gwf_serve_clock_ns.restype = C.c_int64
+        lib.gwf_tune_get.argtypes = [P(Tune)]
+        lib.gwf_tune_set.argtypes = [P(Tune)]
+        lib.gwf_tune_save.argtypes = [C.c_char_p]
+        lib.gwf_tune_load.argtypes = [C.c_char_p]
+        lib.gwf_autotune.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, P(Tune), C.c_char_p, C.c_int64]
         lib.gwf_serve_start.argtypes = [C.c_void_p, P(ServeOpts), P(C.c_void_p)]
         lib.gwf_serve_submit.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                          C.c_void_p, C.c_void_p, C.c_void_p]
@@ -540,3 +565,35 @@ class Server:
             self.stop()
         except Exception:
             pass
+
+
+def tune_get():
+    """The current tuning table as a list of bins (dicts)."""
+    t = Tune()
+    _check(hip_lib().gwf_tune_get(C.byref(t)), "gwf_tune_get")
+    return t.as_list()
+
+
+def tune_set(bins=None):
+    """Install a tuning table (list of bin dicts); None restores the built-in one."""
+    if bins is None:
+        _check(hip_lib().gwf_tune_set(None), "gwf_tune_set")
+    else:
+        t = Tune.from_list(bins)
+        _check(hip_lib().gwf_tune_set(C.byref(t)), "gwf_tune_set")
+
+
+def tune_save(path):
+    _check(hip_lib().gwf_tune_save(path.encode()), "gwf_tune_save")
+
+
+def tune_load(path):
+    _check(hip_lib().gwf_tune_load(path.encode()), "gwf_tune_load")
+
+
+def autotune(dgraph, dreads, max_evals=0):
+    """Run the autotuner on dreads (a workload sample); returns (table, report text)."""
+    t = Tune()
+    buf = C.create_string_buffer(1 << 16)
+    _check(hip_lib().gwf_autotune(dgraph.ptr, dreads.ptr, max_evals, C.byref(t), buf, len(buf)), "gwf_autotune")
+    return t.as_list(), buf.value.decode()

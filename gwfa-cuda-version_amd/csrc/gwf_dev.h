@@ -66,6 +66,7 @@ struct PersistCfg {
 	Caps lds;        // LDS-resident capacities
 	Caps hbm;        // HBM spill capacities
 	size_t lds_bytes;
+	i32 mw;          // LDS sort: at most mw runs of 64 are merged by one rank pass
 };
 
 size_t gwf_ws_bytes(const Caps &c);
@@ -75,13 +76,35 @@ hipError_t gwf_launch_plain(const DGraph &g, const DReads &r, const i32 *read_id
                             u64 *scratch, size_t ws_bytes, i32 *out_ed, i32 *out_status, int64_t *out_counts,
                             hipStream_t st);
 
-// Persistent kernel: cfg.wgs workgroups pull reads order[head..n) through the
-// device counter *head until the queue is empty.
-// beat (may be null): one GwfBeat per workgroup in host-coherent memory.
+// Persistent kernel: cfg.wgs workgroups pull the reads of bin `bin`,
+// order[bin_off[bin] .. bin_off[bin + 1]) (device arrays; order == nullptr:
+// the identity), through the device counter heads[bin] (zero at launch)
+// until the queue is empty.  beat (may be null): one GwfBeat per workgroup
+// in host-coherent memory.
 struct GwfBeat;
-hipError_t gwf_launch_persistent(const DGraph &g, const DReads &r, const i32 *order, i32 n, i32 *head,
-                                 const PersistCfg &cfg, u64 *scratch, size_t ws_bytes, i32 *out_ed,
+hipError_t gwf_launch_persistent(const DGraph &g, const DReads &r, const i32 *order, const i32 *bin_off, i32 bin,
+                                 i32 *heads, const PersistCfg &cfg, u64 *scratch, size_t ws_bytes, i32 *out_ed,
                                  i32 *out_status, int64_t *out_counts, GwfBeat *beat, hipStream_t st);
+
+// ------------------------------------------------------------ device-side binning
+// Tuning bins: a read goes to the first bin whose est_max is at least its
+// predicted wavefront size (~3.2 x predicted ED + 16; the predicted ED is
+// the read's length times the error rate its probe prefix showed).
+#define GWF_MAX_BINS 8
+#define GWF_COST_BUCKETS 64
+struct BinBounds {
+	i32 n_bins;
+	i32 est_max[GWF_MAX_BINS];
+};
+// Order reads for the binned persistent launches, entirely on the device (so
+// the sequence can be captured in a graph): from the probe's EDs, every
+// read's bin and cost (ql + 4 ED^2) are computed; a counting sort over
+// (bin, descending log-cost bucket) fills order[] with each bin's reads,
+// longest first, and bin_off[0..n_bins] with the bins' boundaries.  hist
+// holds n_bins * GWF_COST_BUCKETS counters and key n entries of scratch;
+// heads[0..GWF_MAX_BINS) is cleared for the launches that follow.
+hipError_t gwf_launch_binning(const i32 *probe_ed, const i32 *probe_status, const i32 *ql, i32 n, const BinBounds &bb,
+                              i32 *key, i32 *hist, i32 *order, i32 *bin_off, i32 *heads, hipStream_t st);
 
 // Largest dynamic LDS a workgroup may use on this device.
 size_t gwf_max_lds_bytes();
@@ -137,6 +160,7 @@ struct SrvCtl {
 #define GWF_SRV_STOP (1ull << 63)
 
 struct SrvArgs {
+	i32 mw;                 // LDS sort variant (see PersistCfg)
 	DGraph g;
 	const SrvEntry *ring;   // host-coherent, mask + 1 entries
 	SrvResult *res;         // host-coherent, mask + 1 results
@@ -156,3 +180,9 @@ hipError_t gwf_launch_serve(const SrvArgs &a, const PersistCfg &cfg, hipStream_t
 
 // Resident server workgroups per CU for nw waves and lds_bytes of LDS.
 int gwf_serve_occupancy(int nw, size_t lds_bytes);
+
+// p[0..n) = v (a kernel, so it can be part of a captured graph).
+hipError_t gwf_launch_fill(i32 *p, i32 n, i32 v, hipStream_t st);
+// Set the dynamic-LDS limits of the persistent and server kernels; call once
+// per device before the first launch (and before any graph capture).
+hipError_t gwf_kernels_init();

@@ -347,9 +347,10 @@ __device__ __forceinline__ u64 wave_sort64(u64 x) {
 
 // Sort src[0..n) into dst: every wave sorts runs of 64 in registers (no
 // barriers), then each key's final rank is found by binary search in the
-// other runs (up to 8 runs), or runs are merged pairwise through dst/src.
+// other runs (one pass, when there are at most mw runs), or runs are merged
+// pairwise through dst/src (log2(runs) passes).  mw is a tuning parameter.
 template <int NW>
-__device__ __forceinline__ void sort_runs(u64 *src, i32 n, u64 *dst, const Blk<NW> &b) {
+__device__ __forceinline__ void sort_runs(u64 *src, i32 n, u64 *dst, i32 mw, const Blk<NW> &b) {
 	const i32 k = (n + WAVE - 1) / WAVE;
 	for (i32 c = (i32)b.wid(); c < k; c += NW) {
 		const i32 j = c * WAVE + (i32)lane_id();
@@ -363,7 +364,7 @@ __device__ __forceinline__ void sort_runs(u64 *src, i32 n, u64 *dst, const Blk<N
 		b.sync();
 		return;
 	}
-	if (k <= 8) {
+	if (k <= mw) {
 		for (i32 j = (i32)b.tid(); j < n; j += Blk<NW>::NT) {
 			const u64 x = src[j];
 			const i32 r = j / WAVE;
@@ -473,6 +474,7 @@ struct Bufs {
 	u64 *H0lo, *H0hi, *H1lo, *H1hi;  // forbidden-interval history, double buffered
 	u64 *sort_lds;    // HBM mode: LDS scratch for sorting runs
 	i32 sort_cap;     // its capacity (power of two)
+	i32 mw;           // LDS sort: runs of 64 merged in one rank pass when at most mw of them
 	u64 *sort_tmp;    // HBM mode: merge temp (max(x, t) elements)
 	Caps c;
 };
@@ -501,6 +503,7 @@ __device__ inline Bufs carve(u64 *p, const Caps &c, bool with_tmp) {
 	w.H1hi = p; p += c.h;
 	w.sort_tmp = with_tmp ? p : nullptr;
 	w.sort_lds = nullptr;
+	w.mw = 8;
 	w.sort_cap = 0;
 	return w;
 }
@@ -706,7 +709,7 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 		sort_hbm(w.X, n_x, w.sort_lds, w.sort_cap, w.sort_tmp, b);
 		Xs = w.X;
 	} else {
-		sort_runs(w.X, n_x, w.P, b);
+		sort_runs(w.X, n_x, w.P, w.mw, b);
 		Xs = w.P;
 	}
 
@@ -878,29 +881,35 @@ __device__ __forceinline__ i32 arena_sort_cap(const Caps &lc) {
 
 // Persistent: resident workgroups pull reads from a queue; each read is
 // aligned by NW waves with its arrays in LDS until they outgrow it.
+// The queue of bin `bin` is order[bin_off[bin] .. bin_off[bin + 1]) (device
+// memory, so a device-side ordering pass can fill it); heads[bin] counts the
+// reads taken from it.  order == nullptr: the identity order.
 template <int NW>
-__global__ __launch_bounds__(WAVE *NW) void gwf_persistent_kernel(DGraph g, DReads rd, const i32 *order, i32 n,
-                                                                 i32 *head, Caps lc, Caps hc, u64 *scratch,
-                                                                 u64 ws_elems_per_wg, i32 *out_ed, i32 *out_status,
-                                                                 int64_t *out_counts, GwfBeat *beats) {
+__global__ __launch_bounds__(WAVE *NW) void gwf_persistent_kernel(DGraph g, DReads rd, const i32 *order,
+                                                                 const i32 *bin_off, i32 bin, i32 *heads, Caps lc,
+                                                                 Caps hc, i32 mw, u64 *scratch, u64 ws_elems_per_wg,
+                                                                 i32 *out_ed, i32 *out_status, int64_t *out_counts,
+                                                                 GwfBeat *beats) {
 	extern __shared__ u64 arena[];
 	__shared__ u64 vis[VIS_SLOTS];
 	__shared__ i32 red[NW];
 	__shared__ u64 redu[NW];
 	__shared__ i32 next_q;
 	Blk<NW> b{red, redu};
-	const Bufs lb = carve(arena, lc, false);
+	Bufs lb = carve(arena, lc, false);
+	lb.mw = mw;
 	Bufs hb = carve(scratch + (size_t)blockIdx.x * ws_elems_per_wg, hc, true);
+	const i32 q_lo = bin_off[bin], q_n = bin_off[bin + 1] - q_lo;
 	hb.sort_lds = arena;  // the LDS arena is free once a read has spilled
 	hb.sort_cap = arena_sort_cap(lc);
 	GwfBeat *beat = beats ? beats + blockIdx.x : nullptr;
 	for (;;) {
-		if (b.tid() == 0) next_q = atomicAdd(head, 1);
+		if (b.tid() == 0) next_q = atomicAdd(&heads[bin], 1);
 		b.sync();
 		const i32 qi = next_q;
 		b.sync();
-		if (qi >= n) break;
-		const i32 r = order[qi];
+		if (qi >= q_n) break;
+		const i32 r = order ? order[q_lo + qi] : q_lo + qi;
 		const u64 t0 = wall_clock64();
 		const ReadOut o = align_read<NW, true>(g, rd.seq + rd.off[r], rd.ql[r], (u32)rd.v0[r], lb, hb, b, vis, beat, r);
 		write_out<NW>(o, r, t0, out_ed, out_status, out_counts, b);
@@ -937,7 +946,8 @@ __global__ __launch_bounds__(WAVE *NW) void gwf_serve_kernel(SrvArgs a) {
 	constexpr i32 NT = Blk<NW>::NT;
 	static_assert(sizeof(SrvEntry) == 64, "SrvEntry is eight u64 words");
 	Blk<NW> b{red, redu};
-	const Bufs lb = carve(arena, a.lc, false);
+	Bufs lb = carve(arena, a.lc, false);
+	lb.mw = a.mw;
 	const i32 sort_cap = arena_sort_cap(a.lc);
 	GwfBeat *beat = a.beat ? a.beat + blockIdx.x : nullptr;
 	i32 idle = 0;
@@ -1042,6 +1052,99 @@ __global__ __launch_bounds__(WAVE *NW) void gwf_serve_kernel(SrvArgs a) {
 	}
 }
 
+// ------------------------------------------------------------ device-side binning
+__global__ __launch_bounds__(256) void gwf_bin_zero_kernel(i32 *hist, i32 nh, i32 *heads) {
+	const i32 t = (i32)(blockIdx.x * 256 + threadIdx.x);
+	if (t < nh) hist[t] = 0;
+	if (t < GWF_MAX_BINS) heads[t] = 0;
+}
+
+__global__ __launch_bounds__(256) void gwf_bin_count_kernel(const i32 *probe_ed, const i32 *probe_status,
+                                                           const i32 *ql, i32 n, BinBounds bb, i32 *key, i32 *hist) {
+	const i32 i = (i32)(blockIdx.x * 256 + threadIdx.x);
+	if (i >= n) return;
+	const i32 q = ql[i];
+	const i32 pl = q < 256 ? q : 256;
+	const float e = (probe_status[i] == GWF_ST_OK && pl > 0) ? (float)(probe_ed[i] > 0 ? probe_ed[i] : 0) / (float)pl
+	                                                         : 0.3f;
+	const float ed = e * (float)q;
+	const i32 est = (i32)(3.2f * ed) + 16;
+	i32 bin = bb.n_bins - 1;
+	for (i32 j = 0; j < bb.n_bins; ++j)
+		if (est <= bb.est_max[j]) { bin = j; break; }
+	// cost bucket: two per power of two of ql + 4 ED^2
+	const u64 cost = (u64)q + (u64)(4.0f * ed * ed) + 1;
+	const i32 L = 63 - __clzll((long long)cost);
+	const i32 half = L > 0 ? (i32)((cost >> (L - 1)) & 1) : 0;
+	i32 u = 2 * L + half;
+	u = u > GWF_COST_BUCKETS - 1 ? GWF_COST_BUCKETS - 1 : u;
+	const i32 slot = bin * GWF_COST_BUCKETS + (GWF_COST_BUCKETS - 1 - u);  // longest first within a bin
+	key[i] = slot;
+	atomicAdd(&hist[slot], 1);
+}
+
+// one workgroup: exclusive scan of the (bin, bucket) counts into cursors
+__global__ __launch_bounds__(512) void gwf_bin_scan_kernel(i32 *hist, i32 nh, i32 n_bins, i32 *bin_off) {
+	__shared__ i32 sh[512];
+	const i32 t = (i32)threadIdx.x;
+	const i32 v = t < nh ? hist[t] : 0;
+	sh[t] = v;
+	__syncthreads();
+	for (i32 off = 1; off < 512; off <<= 1) {
+		const i32 x = t >= off ? sh[t - off] : 0;
+		__syncthreads();
+		sh[t] += x;
+		__syncthreads();
+	}
+	const i32 excl = sh[t] - v;
+	if (t < nh) hist[t] = excl;
+	if (t < nh && t % GWF_COST_BUCKETS == 0) bin_off[t / GWF_COST_BUCKETS] = excl;
+	if (t == 0) bin_off[n_bins] = sh[511];
+}
+
+__global__ __launch_bounds__(256) void gwf_bin_scatter_kernel(const i32 *key, i32 n, i32 *cursor, i32 *order) {
+	const i32 i = (i32)(blockIdx.x * 256 + threadIdx.x);
+	if (i >= n) return;
+	order[atomicAdd(&cursor[key[i]], 1)] = i;
+}
+
+__global__ __launch_bounds__(256) void gwf_fill_kernel(i32 *p, i32 n, i32 v) {
+	const i32 t = (i32)(blockIdx.x * 256 + threadIdx.x);
+	if (t < n) p[t] = v;
+}
+
+hipError_t gwf_launch_fill(i32 *p, i32 n, i32 v, hipStream_t st) {
+	if (n <= 0) return hipSuccess;
+	hipLaunchKernelGGL(gwf_fill_kernel, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
+	return hipGetLastError();
+}
+
+// Set every dynamic-LDS limit once, outside any graph capture.
+hipError_t gwf_kernels_init() {
+	const int mx = (int)gwf_max_lds_bytes();
+	hipError_t e = hipSuccess;
+	const void *ks[] = {(const void *)gwf_persistent_kernel<1>, (const void *)gwf_persistent_kernel<2>,
+	                    (const void *)gwf_persistent_kernel<4>, (const void *)gwf_serve_kernel<1>,
+	                    (const void *)gwf_serve_kernel<2>, (const void *)gwf_serve_kernel<4>};
+	for (const void *k : ks)
+		if (e == hipSuccess) e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
+	return e;
+}
+
+hipError_t gwf_launch_binning(const i32 *probe_ed, const i32 *probe_status, const i32 *ql, i32 n, const BinBounds &bb,
+                              i32 *key, i32 *hist, i32 *order, i32 *bin_off, i32 *heads, hipStream_t st) {
+	if (bb.n_bins < 1 || bb.n_bins > GWF_MAX_BINS || n < 0) return hipErrorInvalidValue;
+	const i32 nh = bb.n_bins * GWF_COST_BUCKETS;
+	hipLaunchKernelGGL(gwf_bin_zero_kernel, dim3((nh + 255) / 256), dim3(256), 0, st, hist, nh, heads);
+	if (n > 0)
+		hipLaunchKernelGGL(gwf_bin_count_kernel, dim3((n + 255) / 256), dim3(256), 0, st, probe_ed, probe_status, ql,
+		                   n, bb, key, hist);
+	hipLaunchKernelGGL(gwf_bin_scan_kernel, dim3(1), dim3(512), 0, st, hist, nh, bb.n_bins, bin_off);
+	if (n > 0)
+		hipLaunchKernelGGL(gwf_bin_scatter_kernel, dim3((n + 255) / 256), dim3(256), 0, st, key, n, hist, order);
+	return hipGetLastError();
+}
+
 hipError_t gwf_launch_plain(const DGraph &g, const DReads &r, const i32 *read_idx, i32 n, const Caps &caps,
                             u64 *scratch, size_t ws_bytes, i32 *out_ed, i32 *out_status, int64_t *out_counts,
                             hipStream_t st) {
@@ -1052,30 +1155,17 @@ hipError_t gwf_launch_plain(const DGraph &g, const DReads &r, const i32 *read_id
 }
 
 template <int NW>
-static hipError_t launch_persist(const DGraph &g, const DReads &r, const i32 *order, i32 n, i32 *head,
-                                 const PersistCfg &cfg, u64 *scratch, size_t ws_bytes, i32 *out_ed, i32 *out_status,
-                                 int64_t *out_counts, GwfBeat *beat, hipStream_t st) {
-	static bool attr_set = false;
-	if (!attr_set) {
-		hipError_t e = hipFuncSetAttribute((const void *)gwf_persistent_kernel<NW>,
-		                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)gwf_max_lds_bytes());
-		if (e != hipSuccess) return e;
-		attr_set = true;
-	}
-	hipLaunchKernelGGL(gwf_persistent_kernel<NW>, dim3(cfg.wgs), dim3(WAVE * NW), cfg.lds_bytes, st, g, r, order, n,
-	                   head, cfg.lds, cfg.hbm, scratch, (u64)(ws_bytes / 8), out_ed, out_status, out_counts, beat);
+static hipError_t launch_persist(const DGraph &g, const DReads &r, const i32 *order, const i32 *bin_off, i32 bin,
+                                 i32 *heads, const PersistCfg &cfg, u64 *scratch, size_t ws_bytes, i32 *out_ed,
+                                 i32 *out_status, int64_t *out_counts, GwfBeat *beat, hipStream_t st) {
+	hipLaunchKernelGGL(gwf_persistent_kernel<NW>, dim3(cfg.wgs), dim3(WAVE * NW), cfg.lds_bytes, st, g, r, order,
+	                   bin_off, bin, heads, cfg.lds, cfg.hbm, cfg.mw, scratch, (u64)(ws_bytes / 8), out_ed, out_status,
+	                   out_counts, beat);
 	return hipGetLastError();
 }
 
 template <int NW>
 static hipError_t launch_serve(const SrvArgs &a, const PersistCfg &cfg, hipStream_t st) {
-	static bool attr_set = false;
-	if (!attr_set) {
-		hipError_t e = hipFuncSetAttribute((const void *)gwf_serve_kernel<NW>,
-		                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)gwf_max_lds_bytes());
-		if (e != hipSuccess) return e;
-		attr_set = true;
-	}
 	hipLaunchKernelGGL(gwf_serve_kernel<NW>, dim3(cfg.wgs), dim3(WAVE * NW), cfg.lds_bytes, st, a);
 	return hipGetLastError();
 }
@@ -1089,14 +1179,13 @@ hipError_t gwf_launch_serve(const SrvArgs &a, const PersistCfg &cfg, hipStream_t
 	}
 }
 
-hipError_t gwf_launch_persistent(const DGraph &g, const DReads &r, const i32 *order, i32 n, i32 *head,
-                                 const PersistCfg &cfg, u64 *scratch, size_t ws_bytes, i32 *out_ed, i32 *out_status,
-                                 int64_t *out_counts, GwfBeat *beat, hipStream_t st) {
-	if (n <= 0) return hipSuccess;
+hipError_t gwf_launch_persistent(const DGraph &g, const DReads &r, const i32 *order, const i32 *bin_off, i32 bin,
+                                 i32 *heads, const PersistCfg &cfg, u64 *scratch, size_t ws_bytes, i32 *out_ed,
+                                 i32 *out_status, int64_t *out_counts, GwfBeat *beat, hipStream_t st) {
 	switch (cfg.nw) {
-	case 1: return launch_persist<1>(g, r, order, n, head, cfg, scratch, ws_bytes, out_ed, out_status, out_counts, beat, st);
-	case 2: return launch_persist<2>(g, r, order, n, head, cfg, scratch, ws_bytes, out_ed, out_status, out_counts, beat, st);
-	case 4: return launch_persist<4>(g, r, order, n, head, cfg, scratch, ws_bytes, out_ed, out_status, out_counts, beat, st);
+	case 1: return launch_persist<1>(g, r, order, bin_off, bin, heads, cfg, scratch, ws_bytes, out_ed, out_status, out_counts, beat, st);
+	case 2: return launch_persist<2>(g, r, order, bin_off, bin, heads, cfg, scratch, ws_bytes, out_ed, out_status, out_counts, beat, st);
+	case 4: return launch_persist<4>(g, r, order, bin_off, bin, heads, cfg, scratch, ws_bytes, out_ed, out_status, out_counts, beat, st);
 	default: return hipErrorInvalidValue;
 	}
 }

@@ -79,6 +79,16 @@ struct RunCtx {
 	size_t scratch_bytes = 0;
 	GwfBeat *beat = nullptr;  // host-coherent heartbeat records
 	size_t beat_n = 0;
+	// binned persistent runs
+	i32 *pe = nullptr, *ps = nullptr, *key = nullptr, *ord = nullptr, *hist = nullptr, *boff = nullptr;
+	size_t pe_c = 0, ps_c = 0, key_c = 0, ord_c = 0;
+	hipStream_t sb[GWF_MAX_BINS] = {};
+	hipEvent_t ev_fork = nullptr, ev_join[GWF_MAX_BINS] = {};
+	u64 *scr[GWF_MAX_BINS] = {};
+	size_t scr_c[GWF_MAX_BINS] = {};
+	// graph of the last binned run, replayed while nothing it captured changes
+	hipGraphExec_t gexec = nullptr;
+	std::vector<uintptr_t> gkey;
 };
 RunCtx g_run;
 
@@ -107,6 +117,19 @@ int ctx_prepare(RunCtx &c, size_t n) {
 		HIPCHK(hipDeviceGetAttribute(&c.wall_khz, hipDeviceAttributeWallClockRate, dev));
 		HIPCHK(hipDeviceGetAttribute(&c.n_cu, hipDeviceAttributeMultiprocessorCount, dev));
 		HIPCHK(hipMalloc((void **)&c.head, 256));
+		HIPCHK(hipMalloc((void **)&c.hist, sizeof(i32) * GWF_MAX_BINS * GWF_COST_BUCKETS));
+		HIPCHK(hipMalloc((void **)&c.boff, 256));
+		for (int b = 0; b < GWF_MAX_BINS; ++b) {
+			HIPCHK(hipStreamCreate(&c.sb[b]));
+			HIPCHK(hipEventCreateWithFlags(&c.ev_join[b], hipEventDisableTiming));
+		}
+		HIPCHK(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
+		HIPCHK(gwf_kernels_init());
+		c.pe = c.ps = c.key = c.ord = nullptr;
+		c.pe_c = c.ps_c = c.key_c = c.ord_c = 0;
+		for (int b = 0; b < GWF_MAX_BINS; ++b) c.scr[b] = nullptr, c.scr_c[b] = 0;
+		c.gexec = nullptr;
+		c.gkey.clear();
 		c.dev = dev;
 	}
 	if (c.n_cap < n) {
@@ -155,25 +178,92 @@ Caps lds_caps(i32 m) {
 	return c;
 }
 
-// The tuning table: waves per read and LDS wavefront capacity for a bin of
-// reads, from the largest expected wavefront of the bin.  Overridable with
-// GWF_PERSIST_NW / GWF_PERSIST_M.
-PersistCfg choose_cfg(const RunCtx &cx, int64_t est_states, i32 max_ql) {
+// Launch parameters for waves-per-read nw, LDS wavefront capacity m and
+// sort variant mw (GWF_PERSIST_NW / _M / _WGS override, for tests).
+PersistCfg make_cfg(const RunCtx &cx, i32 nw, i32 m, i32 mw, i32 max_ql, bool env_override = true) {
 	PersistCfg c;
-	// Measured on MI355X (config 2 sweep): LDS capacity beyond ~1024 states
-	// costs more in resident reads than it saves in HBM spills.
-	i32 m = est_states <= 192 ? 256 : est_states <= 512 ? 512 : 1024;
-	i32 nw = m <= 256 ? 1 : m <= 512 ? 2 : 4;
-	nw = (i32)env_int("GWF_PERSIST_NW", nw);
-	m = (i32)env_int("GWF_PERSIST_M", m);
+	if (env_override) {
+		nw = (i32)env_int("GWF_PERSIST_NW", nw);
+		m = (i32)env_int("GWF_PERSIST_M", m);
+	}
 	c.nw = nw;
+	c.mw = mw;
 	c.lds = lds_caps(m);
 	c.lds_bytes = gwf_lds_bytes(c.lds);
 	c.hbm = caps_for(max_ql, 1);
 	const int per_cu = gwf_persist_occupancy(nw, c.lds_bytes);
 	c.wgs = std::max(1, per_cu) * std::max(1, cx.n_cu);
-	c.wgs = (i32)env_int("GWF_PERSIST_WGS", c.wgs);
+	if (env_override) c.wgs = (i32)env_int("GWF_PERSIST_WGS", c.wgs);
 	return c;
+}
+
+// The tuning table: per bin of predicted wavefront size, waves per read,
+// LDS capacity and sort variant.  Built-in values are measured on MI355X
+// (gwf_autotune replaces them for a workload).
+std::mutex g_tune_mu;
+gwf_tune_t g_tune;
+u64 g_tune_version = 1;
+
+gwf_tune_t builtin_tune() {
+	gwf_tune_t t;
+	memset(&t, 0, sizeof(t));
+	const i32 est[] = {160, 480, 1200, 3000, 1 << 30};
+	const i32 nw[] = {1, 2, 4, 4, 4};
+	const i32 m[] = {256, 512, 1024, 1024, 1024};
+	t.n_bins = 5;
+	for (int b = 0; b < 5; ++b) t.bin[b] = {est[b], nw[b], m[b], 8};
+	return t;
+}
+
+int tune_check(const gwf_tune_t &t) {
+	if (t.n_bins < 1 || t.n_bins > GWF_MAX_BINS) return GWF_ERR_ARG;
+	for (int b = 0; b < t.n_bins; ++b) {
+		const gwf_tune_bin_t &x = t.bin[b];
+		if (x.nw != 1 && x.nw != 2 && x.nw != 4) return GWF_ERR_ARG;
+		if (x.lds_states < 64 || gwf_lds_bytes(lds_caps(x.lds_states)) > gwf_max_lds_bytes()) return GWF_ERR_ARG;
+		if (x.merge_runs < 0 || x.merge_runs > 64) return GWF_ERR_ARG;
+		if (b > 0 && x.est_max <= t.bin[b - 1].est_max) return GWF_ERR_ARG;
+	}
+	return GWF_OK;
+}
+
+// Text form, one bin per line:
+//   # comment
+//   bin <est_max> <nw> <lds_states> <merge_runs>
+int gwf_tune_read_file(const char *path, gwf_tune_t *out) {
+	FILE *f = fopen(path, "r");
+	if (!f) return GWF_ERR_ARG;
+	gwf_tune_t t;
+	memset(&t, 0, sizeof(t));
+	char line[256];
+	int rc = GWF_OK;
+	while (fgets(line, sizeof line, f)) {
+		if (line[0] == '#' || line[0] == '\n') continue;
+		gwf_tune_bin_t x;
+		if (sscanf(line, "bin %d %d %d %d", &x.est_max, &x.nw, &x.lds_states, &x.merge_runs) != 4 ||
+		    t.n_bins >= GWF_MAX_BINS) {
+			rc = GWF_ERR_ARG;
+			break;
+		}
+		t.bin[t.n_bins++] = x;
+	}
+	fclose(f);
+	if (rc == GWF_OK) rc = tune_check(t);
+	if (rc == GWF_OK) *out = t;
+	return rc;
+}
+
+gwf_tune_t tune_current(u64 *version) {
+	std::lock_guard<std::mutex> lk(g_tune_mu);
+	if (g_tune.n_bins == 0) {
+		g_tune = builtin_tune();
+		if (const char *f = getenv("GWF_TUNE_FILE")) {
+			gwf_tune_t t;
+			if (gwf_tune_read_file(f, &t) == GWF_OK && tune_check(t) == GWF_OK) g_tune = t;
+		}
+	}
+	if (version) *version = g_tune_version;
+	return g_tune;
 }
 
 std::mutex g_cache_mu;
@@ -478,7 +568,7 @@ bool run_plain(RunCtx &cx, RunAcc &acc, const gwf_dgraph_t *dg, const DReads &r,
 	return true;
 }
 
-// Persistent kernel over `order` (a queue drained by the resident workgroups).
+// Persistent kernel over one host-ordered queue (the autotuner's runs).
 bool run_persistent(RunCtx &cx, RunAcc &acc, const gwf_dgraph_t *dg, const DReads &r, const std::vector<i32> &order,
                     const PersistCfg &cfg) {
 	const i32 n = (i32)order.size();
@@ -486,7 +576,9 @@ bool run_persistent(RunCtx &cx, RunAcc &acc, const gwf_dgraph_t *dg, const DRead
 	const size_t ws = gwf_ws_bytes(cfg.hbm);
 	ACC(grow(&cx.scratch, &cx.scratch_bytes, ws * (size_t)cfg.wgs));
 	ACC(hipMemcpyAsync(cx.idx, order.data(), 4 * (size_t)n, hipMemcpyHostToDevice, cx.s));
-	ACC(hipMemsetAsync(cx.head, 0, sizeof(i32), cx.s));
+	ACC(gwf_launch_fill(cx.boff + 16, 1, 0, cx.s));
+	ACC(gwf_launch_fill(cx.boff + 17, 1, n, cx.s));
+	ACC(gwf_launch_fill(cx.head + 16, 1, 0, cx.s));
 	Heartbeat hb;
 	hb.period_ms = (int)env_int("GWF_HEARTBEAT_MS", 0);
 	if (hb.period_ms > 0) {
@@ -503,9 +595,148 @@ bool run_persistent(RunCtx &cx, RunAcc &acc, const gwf_dgraph_t *dg, const DRead
 		hb.what = "persistent";
 	}
 	return timed_launch(cx, acc, [&] {
-		return gwf_launch_persistent(dg->g, r, cx.idx, n, cx.head, cfg, cx.scratch, ws, cx.ed, cx.status, cx.cnt,
-		                             hb.beat, cx.s);
+		return gwf_launch_persistent(dg->g, r, cx.idx, cx.boff + 16, 0, cx.head + 16, cfg, cx.scratch, ws, cx.ed,
+		                             cx.status, cx.cnt, hb.beat, cx.s);
 	}, &hb);
+}
+
+// The binned run, all on the device: probe every read's prefix, order the
+// reads into per-bin longest-first queues, then one persistent launch per
+// bin with that bin's tuned configuration (on forked streams when their
+// workspaces fit the budget, so each bin's tail overlaps the next bin).
+struct BinnedPlan {
+	PersistCfg probe;
+	PersistCfg bin[GWF_MAX_BINS];
+	BinBounds bb;
+	i32 n = 0;
+	bool concurrent = false;
+	GwfBeat *beat = nullptr;  // heartbeat records: beat_stride per bin (probe: bin n_bins)
+	i32 beat_stride = 0;
+};
+
+hipError_t enqueue_binned(RunCtx &cx, const gwf_dgraph_t *dg, const gwf_dreads_t *dr, const BinnedPlan &p) {
+	hipError_t e;
+#define ENQ(x)                            \
+	do {                                  \
+		if ((e = (x)) != hipSuccess) return e; \
+	} while (0)
+	DReads rp = dr->r;
+	rp.ql = dr->ql_probe;
+	ENQ(gwf_launch_fill(cx.boff + 16, 1, 0, cx.s));
+	ENQ(gwf_launch_fill(cx.boff + 17, 1, p.n, cx.s));
+	ENQ(gwf_launch_fill(cx.head + 16, 1, 0, cx.s));
+	ENQ(gwf_launch_persistent(dg->g, rp, nullptr, cx.boff + 16, 0, cx.head + 16, p.probe, cx.scratch,
+	                          gwf_ws_bytes(p.probe.hbm), cx.pe, cx.ps, cx.cnt,
+	                          p.beat ? p.beat + (size_t)p.bb.n_bins * p.beat_stride : nullptr, cx.s));
+	ENQ(gwf_launch_binning(cx.pe, cx.ps, dr->r.ql, p.n, p.bb, cx.key, cx.hist, cx.ord, cx.boff, cx.head, cx.s));
+	if (p.concurrent) {
+		ENQ(hipEventRecord(cx.ev_fork, cx.s));
+		for (int b = 0; b < p.bb.n_bins; ++b) ENQ(hipStreamWaitEvent(cx.sb[b], cx.ev_fork, 0));
+	}
+	for (int b = p.bb.n_bins - 1; b >= 0; --b) {  // costliest bin first
+		hipStream_t st = p.concurrent ? cx.sb[b] : cx.s;
+		u64 *scr = p.concurrent ? cx.scr[b] : cx.scr[0];
+		ENQ(gwf_launch_persistent(dg->g, dr->r, cx.ord, cx.boff, b, cx.head, p.bin[b], scr,
+		                          gwf_ws_bytes(p.bin[b].hbm), cx.ed, cx.status, cx.cnt,
+		                          p.beat ? p.beat + (size_t)b * p.beat_stride : nullptr, st));
+	}
+	if (p.concurrent) {
+		for (int b = 0; b < p.bb.n_bins; ++b) {
+			ENQ(hipEventRecord(cx.ev_join[b], cx.sb[b]));
+			ENQ(hipStreamWaitEvent(cx.s, cx.ev_join[b], 0));
+		}
+	}
+#undef ENQ
+	return hipSuccess;
+}
+
+bool run_binned(RunCtx &cx, RunAcc &acc, const gwf_dgraph_t *dg, const gwf_dreads_t *dr, i32 maxql, size_t budget,
+                bool graph_ok) {
+	const i32 n = dr->r.n;
+	u64 tv = 0;
+	const gwf_tune_t t = tune_current(&tv);
+	BinnedPlan p;
+	p.n = n;
+	p.probe = make_cfg(cx, 1, 256, 8, std::min(maxql, kProbeLen));
+	p.bb.n_bins = t.n_bins;
+	size_t ws_sum = 0, ws_max = 0;
+	for (int b = 0; b < t.n_bins; ++b) {
+		p.bb.est_max[b] = t.bin[b].est_max;
+		p.bin[b] = make_cfg(cx, t.bin[b].nw, t.bin[b].lds_states, t.bin[b].merge_runs, maxql);
+		const size_t w = gwf_ws_bytes(p.bin[b].hbm) * (size_t)p.bin[b].wgs;
+		ws_sum += w;
+		ws_max = std::max(ws_max, w);
+	}
+	p.concurrent = ws_sum <= budget && env_int("GWF_BIN_STREAMS", 1) != 0;
+	ACC(grow(&cx.scratch, &cx.scratch_bytes, gwf_ws_bytes(p.probe.hbm) * (size_t)p.probe.wgs));
+	ACC(grow(&cx.pe, &cx.pe_c, 4 * (size_t)n));
+	ACC(grow(&cx.ps, &cx.ps_c, 4 * (size_t)n));
+	ACC(grow(&cx.key, &cx.key_c, 4 * (size_t)n));
+	ACC(grow(&cx.ord, &cx.ord_c, 4 * (size_t)n));
+	for (int b = 0; b < (p.concurrent ? t.n_bins : 1); ++b) {
+		const size_t w = p.concurrent ? gwf_ws_bytes(p.bin[b].hbm) * (size_t)p.bin[b].wgs : ws_max;
+		ACC(grow(&cx.scr[b], &cx.scr_c[b], w));
+	}
+	Heartbeat hb;
+	hb.period_ms = (int)env_int("GWF_HEARTBEAT_MS", 0);
+	if (hb.period_ms > 0) {  // heartbeat runs are not captured
+		i32 stride = p.probe.wgs;
+		for (int b = 0; b < t.n_bins; ++b) stride = std::max(stride, p.bin[b].wgs);
+		const size_t need = (size_t)stride * (t.n_bins + 1);
+		if (need > cx.beat_n) {
+			if (cx.beat) (void)hipHostFree(cx.beat);
+			cx.beat = nullptr;
+			cx.beat_n = 0;
+			ACC(hipHostMalloc((void **)&cx.beat, sizeof(GwfBeat) * need, hipHostMallocCoherent | hipHostMallocMapped));
+			cx.beat_n = need;
+		}
+		for (size_t w = 0; w < need; ++w) cx.beat[w].read = -1, cx.beat[w].beat = 0;
+		p.beat = cx.beat;
+		p.beat_stride = stride;
+		hb.beat = cx.beat;
+		hb.n = (i32)need;
+		hb.what = "binned persistent";
+	}
+	const bool use_graph = graph_ok && env_int("GWF_GRAPH", 1) != 0 && !p.beat;
+	acc.st.n_launch += 1 + t.n_bins;
+	if (!use_graph) {
+		ACC(hipEventRecord(cx.e0, cx.s));
+		ACC(enqueue_binned(cx, dg, dr, p));
+		ACC(hipEventRecord(cx.e1, cx.s));
+		if (hb.beat) heartbeat_wait(cx, hb);
+		ACC(hipEventSynchronize(cx.e1));
+		float ms = 0;
+		ACC(hipEventElapsedTime(&ms, cx.e0, cx.e1));
+		acc.st.kernel_ms += ms;
+		return true;
+	}
+	// everything the captured launches point at or were sized by
+	std::vector<uintptr_t> key = {(uintptr_t)dg, (uintptr_t)dr, (uintptr_t)n, (uintptr_t)tv, (uintptr_t)maxql,
+	                              (uintptr_t)p.concurrent, (uintptr_t)cx.ed, (uintptr_t)cx.status, (uintptr_t)cx.cnt,
+	                              (uintptr_t)cx.scratch, (uintptr_t)cx.pe, (uintptr_t)cx.ps, (uintptr_t)cx.key,
+	                              (uintptr_t)cx.ord};
+	for (int b = 0; b < GWF_MAX_BINS; ++b) key.push_back((uintptr_t)cx.scr[b]);
+	for (int b = 0; b < t.n_bins; ++b) {
+		key.push_back((uintptr_t)p.bin[b].nw);
+		key.push_back((uintptr_t)p.bin[b].wgs);
+		key.push_back((uintptr_t)p.bin[b].lds_bytes);
+	}
+	if (!cx.gexec || cx.gkey != key) {
+		if (cx.gexec) (void)hipGraphExecDestroy(cx.gexec);
+		cx.gexec = nullptr;
+		cx.gkey.clear();
+		ACC(hipStreamBeginCapture(cx.s, hipStreamCaptureModeThreadLocal));
+		const hipError_t e = enqueue_binned(cx, dg, dr, p);
+		hipGraph_t g = nullptr;
+		const hipError_t e2 = hipStreamEndCapture(cx.s, &g);
+		ACC(e);
+		ACC(e2);
+		const hipError_t e3 = hipGraphInstantiate(&cx.gexec, g, nullptr, nullptr, 0);
+		(void)hipGraphDestroy(g);
+		ACC(e3);
+		cx.gkey = key;
+	}
+	return timed_launch(cx, acc, [&] { return hipGraphLaunch(cx.gexec, cx.s); });
 }
 
 #undef ACC
@@ -548,33 +779,9 @@ int gwf_ed_run_timed(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t v1, uint32_t ma
 	bool ok = true;
 
 	if (kernel == GWF_KERNEL_PERSISTENT) {
-		// probe: align the first kProbeLen bases of every read to estimate its error rate
-		std::vector<i32> ident(n), probe(n);
-		for (i32 i = 0; i < n; ++i) ident[i] = i;
-		DReads rp = dr->r;
-		rp.ql = dr->ql_probe;
 		i32 maxql = 0;
 		for (i32 q : dr->ql) maxql = std::max(maxql, q);
-		PersistCfg pc = choose_cfg(cx, 64, std::min(maxql, kProbeLen));
-		ok = run_persistent(cx, acc, dg, rp, ident, pc);
-		if (ok) ok = hipMemcpy(probe.data(), cx.ed, 4 * (size_t)n, hipMemcpyDeviceToHost) == hipSuccess;
-		if (ok) ok = hipMemcpy(status.data(), cx.status, 4 * (size_t)n, hipMemcpyDeviceToHost) == hipSuccess;
-		// cost ~ ql + c * ED^2 (the wavefront of score s holds ~2s+1 diagonals)
-		std::vector<double> cost(n);
-		int64_t est_max = 0;
-		for (i32 i = 0; i < n && ok; ++i) {
-			const i32 pl = std::min(dr->ql[i], kProbeLen);
-			const double e = (status[i] == GWF_ST_OK && pl > 0) ? (double)std::max(probe[i], 0) / pl : 0.3;
-			const double ed = e * dr->ql[i];
-			cost[i] = dr->ql[i] + 4.0 * ed * ed;
-			est_max = std::max<int64_t>(est_max, (int64_t)(3.2 * ed) + 16);
-		}
-		std::vector<i32> order(ident);
-		std::stable_sort(order.begin(), order.end(), [&](i32 x, i32 y) { return cost[x] > cost[y]; });
-		if (ok) {
-			const PersistCfg cfg = choose_cfg(cx, est_max, maxql);
-			ok = run_persistent(cx, acc, dg, dr->r, order, cfg);
-		}
+		ok = run_binned(cx, acc, dg, dr, maxql, budget, true);
 		if (ok) ok = hipMemcpy(status.data(), cx.status, 4 * (size_t)n, hipMemcpyDeviceToHost) == hipSuccess;
 		for (i32 i = 0; i < n && ok; ++i)
 			if (status[i] == GWF_ST_OVERFLOW) todo.push_back(i);
@@ -657,6 +864,153 @@ int gwf_ed_batch(gwf_dgraph_t *dg, int32_t n, const char *const *q, const int32_
 	rc = gwf_ed_run(dg, dr, v1, max_lag, kernel, out, stats);
 	gwf_dreads_free(dr);
 	return rc;
+}
+
+int gwf_autotune(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t max_evals, gwf_tune_t *out, char *report,
+                 int64_t report_len) {
+	static_assert(GWF_MAX_BINS == GWF_TUNE_MAX_BINS, "bin tables agree");
+	if (!dg || !dr || max_evals < 0) { set_err("gwf_autotune: bad argument"); return GWF_ERR_ARG; }
+	if (max_evals == 0) max_evals = 64;
+	const i32 n = dr->r.n;
+	std::string rep;
+	char line[320];
+	gwf_tune_t best_t = tune_current(nullptr);
+	if (n > 0) {
+		RunCtx &cx = g_run;
+		std::lock_guard<std::mutex> lock(cx.mu);
+		int rc = ctx_prepare(cx, (size_t)n);
+		if (rc) return rc;
+		i32 maxql = 0;
+		for (i32 q : dr->ql) maxql = std::max(maxql, q);
+		size_t free_b = 0, tot_b = 0;
+		if (hipMemGetInfo(&free_b, &tot_b) != hipSuccess) { set_err("hipMemGetInfo failed"); return GWF_ERR_HIP; }
+		const size_t budget = std::min<size_t>(free_b / 2, (size_t)32 << 30);
+		RunAcc acc;
+		memset(&acc.st, 0, sizeof(acc.st));
+		int evals = 0;
+		std::vector<i32> status(n);
+		// one end-to-end binned run of dr with table t (eager, not captured)
+		auto eval = [&](const gwf_tune_t &t, const char *what) -> double {
+			if (evals >= max_evals) return 1e30;
+			{
+				std::lock_guard<std::mutex> lk(g_tune_mu);
+				g_tune = t;
+				g_tune_version += 1;
+			}
+			const double before = acc.st.kernel_ms;
+			if (!run_binned(cx, acc, dg, dr, maxql, budget, false)) return -1.0;
+			if (hipMemcpy(status.data(), cx.status, 4 * (size_t)n, hipMemcpyDeviceToHost) != hipSuccess) return -1.0;
+			i32 over = 0;
+			for (i32 x : status) over += x != GWF_ST_OK;
+			const double ms = acc.st.kernel_ms - before + (over ? 1e6 : 0.0);  // overflowing tables lose
+			++evals;
+			std::string desc;
+			for (int b = 0; b < t.n_bins; ++b) {
+				char d[64];
+				snprintf(d, sizeof d, "%s(%d,%d,%d)", b ? " " : "", t.bin[b].nw, t.bin[b].lds_states, t.bin[b].merge_runs);
+				desc += d;
+			}
+			snprintf(line, sizeof line, "eval %2d %-14s %9.3f ms  bins (nw,lds,mw): %s\n", evals, what, ms, desc.c_str());
+			rep += line;
+			return ms;
+		};
+		auto uniform = [](gwf_tune_t t, const gwf_tune_bin_t &c) {
+			for (int b = 0; b < t.n_bins; ++b) {
+				const i32 e = t.bin[b].est_max;
+				t.bin[b] = c;
+				t.bin[b].est_max = e;
+			}
+			return t;
+		};
+		const gwf_tune_t base = builtin_tune();
+		static const gwf_tune_bin_t cands[] = {{0, 4, 1024, 8}, {0, 4, 2048, 8}, {0, 4, 512, 8}, {0, 4, 256, 8},
+		                                       {0, 2, 512, 8},  {0, 2, 256, 8},  {0, 2, 128, 8}, {0, 1, 256, 8},
+		                                       {0, 1, 128, 8}};
+		gwf_tune_t one;  // a single bin: one queue, one launch
+		memset(&one, 0, sizeof(one));
+		one.n_bins = 1;
+		one.bin[0].est_max = 1 << 30;
+		const double keep = 0.98;  // a change must gain 2% (run-to-run noise is ~1-2%)
+		// 1. the best single configuration, as one bin
+		double best = 1e30;
+		gwf_tune_bin_t best_c = cands[0];
+		(void)eval(uniform(one, cands[0]), "warm-up");
+		for (const gwf_tune_bin_t &c : cands) {
+			const double ms = eval(uniform(one, c), "one bin");
+			if (ms < 0) { set_err("gwf_autotune: run failed: %s", hipGetErrorString(acc.err)); return GWF_ERR_HIP; }
+			if (ms < best) best = ms, best_t = uniform(one, c), best_c = c;
+		}
+		// 2. the built-in bins, all with that configuration, then per bin,
+		// costliest first, each other configuration for that bin alone
+		gwf_tune_t cur = uniform(base, best_c);
+		double cur_ms = eval(cur, "bins");
+		for (int b = base.n_bins - 1; b >= 0 && cur_ms >= 0; --b) {
+			for (const gwf_tune_bin_t &c : cands) {
+				if (c.nw == cur.bin[b].nw && c.lds_states == cur.bin[b].lds_states) continue;
+				gwf_tune_t t = cur;
+				t.bin[b].nw = c.nw, t.bin[b].lds_states = c.lds_states;
+				const double ms = eval(t, "bin config");
+				if (ms >= 0 && ms < cur_ms * keep) cur_ms = ms, cur = t;
+			}
+		}
+		if (cur_ms >= 0 && cur_ms < best * keep) {  // confirm: bins beat one bin on a second run too
+			const double again = eval(cur, "confirm bins");
+			const double one_again = eval(best_t, "confirm one");
+			if (again >= 0 && one_again >= 0 && again < one_again * keep) best = again, best_t = cur;
+		}
+		// 3. the sort variant, for the whole table
+		static const i32 mws[] = {0, 2, 4, 16, 32};
+		for (i32 mw : mws) {
+			gwf_tune_t t = best_t;
+			for (int b = 0; b < t.n_bins; ++b) t.bin[b].merge_runs = mw;
+			const double ms = eval(t, "sort variant");
+			if (ms >= 0 && ms < best * keep) best = ms, best_t = t;
+		}
+		snprintf(line, sizeof line, "chose %.3f ms after %d runs of %d reads\n", best, evals, n);
+		rep += line;
+	}
+	if (gwf_tune_set(&best_t) != GWF_OK) return GWF_ERR_STATE;
+	if (out) *out = best_t;
+	if (report && report_len > 0) {
+		const size_t k = std::min((size_t)report_len - 1, rep.size());
+		memcpy(report, rep.data(), k);
+		report[k] = 0;
+	}
+	return GWF_OK;
+}
+
+int gwf_tune_get(gwf_tune_t *out) {
+	if (!out) { set_err("gwf_tune_get: NULL argument"); return GWF_ERR_ARG; }
+	*out = tune_current(nullptr);
+	return GWF_OK;
+}
+
+int gwf_tune_set(const gwf_tune_t *t) {
+	const gwf_tune_t x = t ? *t : builtin_tune();
+	if (tune_check(x) != GWF_OK) { set_err("gwf_tune_set: invalid table"); return GWF_ERR_ARG; }
+	std::lock_guard<std::mutex> lk(g_tune_mu);
+	g_tune = x;
+	g_tune_version += 1;
+	return GWF_OK;
+}
+
+int gwf_tune_save(const char *path) {
+	if (!path) { set_err("gwf_tune_save: NULL path"); return GWF_ERR_ARG; }
+	const gwf_tune_t t = tune_current(nullptr);
+	FILE *f = fopen(path, "w");
+	if (!f) { set_err("gwf_tune_save: cannot write %s", path); return GWF_ERR_ARG; }
+	fprintf(f, "# gwfa tuning table: bin <est_max> <nw> <lds_states> <merge_runs>\n");
+	for (int b = 0; b < t.n_bins; ++b)
+		fprintf(f, "bin %d %d %d %d\n", t.bin[b].est_max, t.bin[b].nw, t.bin[b].lds_states, t.bin[b].merge_runs);
+	fclose(f);
+	return GWF_OK;
+}
+
+int gwf_tune_load(const char *path) {
+	if (!path) { set_err("gwf_tune_load: NULL path"); return GWF_ERR_ARG; }
+	gwf_tune_t t;
+	if (gwf_tune_read_file(path, &t) != GWF_OK) { set_err("gwf_tune_load: %s is not a valid table", path); return GWF_ERR_ARG; }
+	return gwf_tune_set(&t);
 }
 
 int32_t gwf_ed(const gwf_graph_t *g, int32_t ql, const char *q, int32_t v0, int32_t v1, uint32_t max_lag,

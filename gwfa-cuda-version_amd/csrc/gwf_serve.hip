@@ -345,6 +345,7 @@ bool gwf_serve_s::device_check(int64_t now) {
 		a.beat = beat_d;
 		a.mask = (u64)o.ring_slots - 1;
 		a.lc = cfg.lds;
+		a.mw = cfg.mw;
 		a.idle_polls = 40000;  // ~5 s of empty polls: the loop ends by itself if the host stops feeding it
 		for (i32 w = 0; w < cfg.wgs; ++w) beat_h[w].read = -1;
 		const hipError_t e = gwf_launch_serve(a, cfg, stream);
@@ -531,6 +532,8 @@ int gwf_serve_start(gwf_dgraph_t *dg, const gwf_serve_opts_t *opts, gwf_serve_t 
 	}
 	c.wgs = o.wgs > 0 ? o.wgs : std::max(1, gwf_serve_occupancy(c.nw, c.lds_bytes)) * std::max(1, n_cu);
 	c.hbm = gwf_caps_for(1, 1);  // unused: each request brings its own workspace capacities
+	c.mw = 8;
+	if (e == hipSuccess) e = gwf_kernels_init();
 	if (e == hipSuccess) e = hipStreamCreate(&s->stream);
 	if (e == hipSuccess) e = host_coherent(&s->ctl_h, &s->ctl_d, 1);
 	if (e == hipSuccess) e = host_coherent(&s->ring_h, &s->ring_d, (size_t)o.ring_slots);

@@ -192,6 +192,49 @@ int gwf_ed_batch(gwf_dgraph_t *dg, int32_t n, const char *const *q, const int32_
 /* Text of the last error seen by this thread ("" if none). */
 const char *gwf_last_error(void);
 
+/* ---- additions: length/error bins and their tuning ------------------------ */
+/* GWF_KERNEL_PERSISTENT runs entirely on the device: every read's first 256
+ * bases are aligned (a probe), the read's predicted wavefront size
+ * (~3.2 x predicted ED + 16, predicted ED = ql x probe ED / probe length)
+ * selects a bin, and each bin's reads are aligned longest-first by one
+ * persistent launch configured from the table below.  Steady-state runs
+ * (same graph, reads and table) replay a captured hipGraph (GWF_GRAPH=0
+ * disables capture).  No counterpart in the reference. */
+
+#define GWF_TUNE_MAX_BINS 8
+typedef struct {
+	int32_t est_max;     /* reads with predicted wavefront size <= est_max (first match) */
+	int32_t nw;          /* 64-lane waves per read: 1, 2 or 4 */
+	int32_t lds_states;  /* LDS wavefront capacity per read (larger spills to HBM) */
+	int32_t merge_runs;  /* sort variant: <= merge_runs sorted runs of 64 are merged in one
+	                        rank pass, more by pairwise merging (0: always pairwise) */
+} gwf_tune_bin_t;
+typedef struct {
+	int32_t n_bins;
+	gwf_tune_bin_t bin[GWF_TUNE_MAX_BINS];
+} gwf_tune_t;
+
+int gwf_tune_get(gwf_tune_t *out);
+/* Replace the process-wide table (NULL: the built-in one).  est_max must
+ * increase from bin to bin; the last bin takes every larger read. */
+int gwf_tune_set(const gwf_tune_t *t);
+/* Text file: lines "bin <est_max> <nw> <lds_states> <merge_runs>".  The
+ * table in GWF_TUNE_FILE, if set, is loaded at first use. */
+int gwf_tune_save(const char *path);
+int gwf_tune_load(const char *path);
+
+/* The autotuner: times complete binned runs of dr (the workload, or a
+ * sample large enough that no single read dominates) under candidate
+ * tables: first every configuration (waves per read x LDS capacity) as a
+ * single bin; then the built-in bin boundaries with that configuration and,
+ * costliest bin first, each other configuration for that bin alone (kept
+ * only if the binned table beats the single bin again on a second run);
+ * then the sort variant.  A change is kept when it is >= 2% faster.  At most
+ * max_evals runs (0: 64).  Installs the best table and copies it to out (may
+ * be NULL); report (may be NULL) receives one line per run. */
+int gwf_autotune(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t max_evals, gwf_tune_t *out,
+                 char *report, int64_t report_len);
+
 /* ---- additions: serving (continuous batching) ----------------------------- */
 /* No counterpart in the reference, whose driver aligns one fixed batch
  * (main.cu:454-750).  A server keeps the device graph resident and runs a

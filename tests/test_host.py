@@ -149,3 +149,33 @@ def test_serve_start_fails_cleanly_without_device(gw):
     assert lib.gwf_serve_start(ctypes.c_void_p(1), None, ctypes.byref(srv)) == -5   # GWF_ERR_NODEV
     assert lib.gwf_serve_start(None, None, ctypes.byref(srv)) == -2                 # GWF_ERR_ARG
     assert lib.gwf_serve_stop(None) == 0
+
+
+def test_tuning_table_roundtrip(gw, tmp_path):
+    """Tuning tables: get/set/save/load through the C-ABI (no device needed)."""
+    base = gw.tune_get()
+    assert 1 <= len(base) <= 8
+    try:
+        mine = [dict(est_max=100, nw=1, lds_states=128, merge_runs=0),
+                dict(est_max=1000, nw=2, lds_states=512, merge_runs=16),
+                dict(est_max=1 << 30, nw=4, lds_states=2048, merge_runs=32)]
+        gw.tune_set(mine)
+        assert gw.tune_get() == mine
+        f = str(tmp_path / "t.txt")
+        gw.tune_save(f)
+        gw.tune_set(None)
+        assert gw.tune_get() == base
+        gw.tune_load(f)
+        assert gw.tune_get() == mine
+        for bad in ([dict(est_max=10, nw=3, lds_states=256, merge_runs=8)],              # nw
+                    [dict(est_max=10, nw=1, lds_states=1 << 20, merge_runs=8)],          # LDS too large
+                    [dict(est_max=50, nw=1, lds_states=256, merge_runs=8),
+                     dict(est_max=40, nw=1, lds_states=256, merge_runs=8)]):             # not increasing
+            with pytest.raises(gw.GwfError):
+                gw.tune_set(bad)
+        (tmp_path / "bad.txt").write_text("bin 10 1 256\n")
+        with pytest.raises(gw.GwfError):
+            gw.tune_load(str(tmp_path / "bad.txt"))
+        assert gw.tune_get() == mine
+    finally:
+        gw.tune_set(None)
