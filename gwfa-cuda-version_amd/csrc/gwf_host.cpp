@@ -1,13 +1,17 @@
 // gwf_host.cpp — libgwfa_host.so: GFA/FASTA readers, graph printers and the
 // seeded synthetic graph/read generator.  Host C++ only (no HIP).
 //
-// GFA semantics follow the reference's GFA layer: S/L line parsing
-// (gfa-io.c:115-336), complement arcs (gfa_fix_symm_add, gfa-base.c:303),
-// removal of duplicate links, and the conversion to a gwf graph rooted at the
-// start segment with reverse-complemented `-` strands and arc overlap = the
-// target-side overlap `ow` (main.cu:294-345).  The vertex order is a
-// topological order rooted at the start vertex (Kahn's algorithm over the
-// reachable subgraph); edit distances do not depend on vertex numbering.
+// The GFA reader restates the reference's GFA layer step by step, so that the
+// gwf graph (vertex order, lengths, sequences and arcs) is the one the
+// reference builds: S/L/FASTA line parsing (gfa_parse_S, gfa_parse_L,
+// gfa-io.c:115-336, 354-427), gfa_finalize (gfa-base.c:455-466: segments
+// without a sequence dropped, arcs sorted, semi-specified overlaps filled,
+// missing complement arcs added, overlaps clamped and arc lengths set, deleted
+// arcs removed), the subgraph extraction gfa_sub_from (gfa-sub.c:41-272: a
+// Kahn-style order driven by an ordered set keyed on (forced flag,
+// unresolved in-arcs, distance, vertex)) and the conversion gwf_gfa2gwf
+// (main.cu:294-345: '-' strands reverse-complemented, arc overlap = the
+// target-side overlap).  Multi-arcs are kept, as there.
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -18,6 +22,8 @@
 #include <unordered_map>
 #include <algorithm>
 #include <deque>
+#include <memory>
+#include <set>
 #include <zlib.h>
 #include "../../include/gwfa.h"
 #include "../../include/gwfa_host.h"
@@ -272,30 +278,50 @@ bool sample_read(const gwf_graph_t *g, const Adj &adj, Rng &r, uint32_t v, int32
 }
 
 // ------------------------------------------------------- GFA parsing
-struct GfaSeg { std::string name, seq; int64_t ln = -1; bool has_s = false; };
-struct GfaArc { uint32_t v, w; int32_t ov, ow; };
+// A segment and an arc as the reference's GFA layer holds them (gfa.h:33-67).
+constexpr int32_t kNoOvl = INT32_MAX;  // overlap not given
+struct GfaSeg {
+	std::string name, seq;
+	int64_t len = 0;
+	bool has_seq = false, del = false;
+};
+struct GfaArc {
+	uint32_t v, w;        // oriented vertices: segment << 1 | strand
+	uint32_t lv = 0;      // len(v) - ov, set once overlaps are final
+	int32_t ov, ow;       // overlap on v's side and on w's side
+	uint32_t link = 0;    // pairs an arc with its complement
+	bool del = false, comp = false;
+	uint64_t key() const { return (uint64_t)v << 32 | lv; }
+};
 
-bool parse_overlap(const char *q, int32_t &ov, int32_t &ow) {
-	if (*q == 0) { ov = ow = 0; return true; }
-	if (*q == '*') { ov = ow = 0; return true; }
-	if (isdigit((unsigned char)*q)) {
-		char *e;
-		long x = strtol(q, &e, 10);
-		if (*e == ':') { ov = (int32_t)x; ow = isdigit((unsigned char)e[1]) ? (int32_t)strtol(e + 1, 0, 10) : 0; return true; }
-		if (isupper((unsigned char)*e)) {  // CIGAR
-			ov = ow = 0;
-			const char *p = q;
-			while (isdigit((unsigned char)*p)) {
-				long l = strtol(p, (char **)&p, 10);
-				if (*p == 'M' || *p == 'D' || *p == 'N') ov += (int32_t)l;
-				if (*p == 'M' || *p == 'I' || *p == 'S') ow += (int32_t)l;
-				++p;
-			}
-			return true;
-		}
-		if (*e == 0) { ov = ow = (int32_t)x; return true; }
+// The overlap field of an L line (gfa_parse_L, gfa-io.c:222-300): "*" is 0;
+// "ov:ow" or ":ow"; a CIGAR sums M/D/N into ov and M/I/S into ow; anything
+// else (or no field) gives 0.
+void parse_overlap(const char *q, int32_t &ov, int32_t &ow) {
+	ov = ow = 0;
+	if (*q == '*' || *q == 0) return;
+	if (*q == ':') {
+		ov = kNoOvl;
+		ow = isdigit((unsigned char)q[1]) ? (int32_t)strtol(q + 1, nullptr, 10) : kNoOvl;
+		return;
 	}
-	return false;
+	if (!isdigit((unsigned char)*q)) return;
+	char *r;
+	const long x = strtol(q, &r, 10);
+	if (isupper((unsigned char)*r)) {
+		const char *p = q;
+		do {
+			char *e;
+			const long l = strtol(p, &e, 10);
+			p = e;
+			if (*p == 'M' || *p == 'D' || *p == 'N') ov += (int32_t)l;
+			if (*p == 'M' || *p == 'I' || *p == 'S') ow += (int32_t)l;
+			++p;
+		} while (isdigit((unsigned char)*p));
+	} else if (*r == ':') {
+		ov = (int32_t)x;
+		ow = isdigit((unsigned char)r[1]) ? (int32_t)strtol(r + 1, nullptr, 10) : kNoOvl;
+	}
 }
 
 struct LineReader {
@@ -451,155 +477,308 @@ int gwf_fasta_load(const char *path, int32_t *n_out, char **seqs_out, int64_t **
 	return GWF_OK;
 }
 
+namespace {
+
+// The graph as the reference's GFA layer leaves it after gfa_read.
+struct GfaGraph {
+	std::vector<GfaSeg> seg;
+	std::unordered_map<std::string, uint32_t> id;
+	std::vector<GfaArc> arc;
+	std::vector<uint64_t> idx;  // per oriented vertex: first arc << 32 | count
+	uint32_t add_seg(const std::string &nm) {  // gfa_add_seg
+		auto it = id.find(nm);
+		if (it != id.end()) return it->second;
+		seg.emplace_back();
+		seg.back().name = nm;
+		id.emplace(nm, (uint32_t)seg.size() - 1);
+		return (uint32_t)seg.size() - 1;
+	}
+	uint32_t n(uint32_t v) const { return (uint32_t)idx[v]; }
+	uint32_t first(uint32_t v) const { return (uint32_t)(idx[v] >> 32); }
+	void sort_index() {  // arcs by (v, lv); ties keep their order
+		std::stable_sort(arc.begin(), arc.end(), [](const GfaArc &a, const GfaArc &b) { return a.key() < b.key(); });
+		index();
+	}
+	void index() {
+		idx.assign(seg.size() * 2, 0);
+		size_t last = 0;
+		for (size_t i = 1; i <= arc.size(); ++i)
+			if (i == arc.size() || arc[i - 1].v != arc[i].v) {
+				idx[arc[i - 1].v] = (uint64_t)last << 32 | (i - last);
+				last = i;
+			}
+	}
+	void finalize();
+};
+
+void GfaGraph::finalize() {  // gfa_finalize, gfa-base.c:455-466
+	for (GfaSeg &s : seg) s.del = s.len == 0;  // gfa_fix_no_seg
+	if (seg.empty()) return;
+	sort_index();
+	const uint32_t nv = (uint32_t)seg.size() * 2;
+	// gfa_fix_semi_arc: an overlap given on one side only is taken from the
+	// complement arc when there is exactly one consistent one; else dropped
+	for (uint32_t v = 0; v < nv; ++v)
+		for (uint32_t i = first(v), e = first(v) + n(v); i < e; ++i) {
+			GfaArc &a = arc[i];
+			if (a.del || (a.ow != kNoOvl && a.ov != kNoOvl)) continue;
+			const uint32_t w = a.w ^ 1;
+			int c = 0;
+			int64_t jv = -1;
+			for (uint32_t j = first(w), f = first(w) + n(w); j < f; ++j)
+				if (!arc[j].del && arc[j].w == (v ^ 1)) ++c, jv = j;
+			bool multi = false;
+			if (c == 1) {
+				if (a.ov != kNoOvl && arc[jv].ow != kNoOvl && a.ov != arc[jv].ow) multi = true;
+				if (a.ow != kNoOvl && arc[jv].ov != kNoOvl && a.ow != arc[jv].ov) multi = true;
+			}
+			if (c == 1 && !multi) {
+				if (arc[jv].ov != kNoOvl) a.ow = arc[jv].ov;
+				if (arc[jv].ow != kNoOvl) a.ov = arc[jv].ow;
+			} else {
+				a.del = true;
+			}
+		}
+	// gfa_fix_symm_add: every arc needs its complement w^1 -> v^1 with the
+	// overlaps swapped; missing ones are appended (and not indexed until the
+	// final sort, so later searches do not see them)
+	for (uint32_t v = 0; v < nv; ++v)
+		for (uint32_t i = first(v), e = first(v) + n(v); i < e; ++i) {
+			if (arc[i].del || arc[i].comp) continue;
+			const uint32_t w = arc[i].w ^ 1;
+			bool found = false;
+			for (uint32_t j = first(w), f = first(w) + n(w); j < f; ++j) {
+				GfaArc &b = arc[j];
+				if (b.del || b.comp) continue;
+				if (b.w == (v ^ 1) && b.ov == arc[i].ow && b.ow == arc[i].ov) {
+					b.comp = true;
+					b.link = arc[i].link;
+					found = true;
+					break;
+				}
+			}
+			if (!found) {
+				GfaArc c;
+				c.v = arc[i].w ^ 1, c.w = v ^ 1, c.ov = arc[i].ow, c.ow = arc[i].ov;
+				c.link = arc[i].link, c.comp = true;
+				arc.push_back(c);
+			}
+		}
+	// gfa_fix_arc_len: clamp overlaps to the segment, drop arcs of dropped
+	// segments, lv = len(v) - ov
+	for (GfaArc &a : arc) {
+		const GfaSeg &sv = seg[a.v >> 1];
+		if (!sv.del && sv.len < a.ov) a.ov = (int32_t)sv.len;
+		if (sv.del || seg[a.w >> 1].del) a.del = true;
+		else a.lv = (uint32_t)(sv.len - a.ov);
+	}
+	// gfa_cleanup: remove deleted arcs, sort, index
+	arc.erase(std::remove_if(arc.begin(), arc.end(), [&](const GfaArc &a) {
+		return a.del || seg[a.v >> 1].del || seg[a.w >> 1].del;
+	}), arc.end());
+	sort_index();
+}
+
+// gfa_sub_from (gfa-sub.c:41-272) with an unbounded radius: the vertices
+// reachable from v0 in the order the reference emits them, and per vertex
+// the arcs into that set as (new target index << 32 | arc index), sorted.
+struct SubNode {
+	uint64_t xnd;     // bit 63: not yet forced; bits 32-62: unresolved in-arcs; bits 0-31: distance
+	uint32_t v;
+	uint32_t pos = 0; // position in the output order
+	bool in_tree = true, forced = false;
+};
+
+void extract_sub(const GfaGraph &g, uint32_t v0, std::vector<uint32_t> &order, std::vector<uint64_t> &sub_arc,
+                 std::vector<uint32_t> &sub_off) {
+	auto tn_n = [](const SubNode *p) { return (uint32_t)(p->xnd << 1 >> 33); };
+	std::unordered_map<uint32_t, SubNode *> h;
+	std::vector<std::unique_ptr<SubNode>> pool;
+	std::set<std::pair<uint64_t, uint32_t>> tree;  // (xnd, v): the order of tn_cmp
+	auto gen = [&](uint32_t v, uint32_t d) {
+		pool.emplace_back(new SubNode());
+		SubNode *p = pool.back().get();
+		p->v = v;
+		p->xnd = 1ull << 63 | (uint64_t)g.n(v ^ 1) << 32 | d;
+		h[v] = p;
+		return p;
+	};
+	auto find = [&](uint32_t v) -> SubNode * {
+		auto it = h.find(v);
+		return it == h.end() ? nullptr : it->second;
+	};
+	{
+		SubNode *p = gen(v0, 0);
+		tree.insert({p->xnd, p->v});
+	}
+	std::vector<SubNode *> L;
+	int32_t orphan_inv = 0;
+	while (!tree.empty()) {
+		SubNode *q = nullptr;
+		SubNode *r = find(tree.begin()->second);
+		if (orphan_inv) {  // first the vertices whose complement has already left the set
+			for (auto it = tree.begin(); it != tree.end(); ++it) {
+				SubNode *x = find(it->second);
+				SubNode *c = find(x->v ^ 1);
+				if (c && !c->in_tree) {
+					--orphan_inv;
+					q = x;
+					tree.erase(it);
+					break;
+				}
+			}
+		} else if (tn_n(r) > 0) {
+			// the smallest vertex still has unresolved in-arcs: unless every
+			// in-arc comes from a vertex whose complement alone was emitted,
+			// clear every vertex's "not forced" bit
+			const uint32_t u = r->v ^ 1;
+			uint32_t i = 0;
+			for (; i < g.n(u); ++i) {
+				const GfaArc &a = g.arc[g.first(u) + i];
+				SubNode *k1 = find(a.w ^ 1), *k2 = find(a.w);
+				if (!((k1 == nullptr && k2 && !k2->in_tree) || (k2 == nullptr && k1 && !k1->in_tree))) break;
+			}
+			if (i < g.n(u)) {
+				std::vector<SubNode *> all;
+				for (const auto &e : tree) all.push_back(find(e.second));
+				tree.clear();
+				for (SubNode *x : all) {
+					x->xnd &= ~(1ull << 63);
+					tree.insert({x->xnd, x->v});
+				}
+			}
+		}
+		if (!q) {
+			q = find(tree.begin()->second);
+			tree.erase(tree.begin());
+		}
+		q->forced = tn_n(q) > 0 || (q->xnd >> 63) == 0;
+		q->in_tree = false;
+		L.push_back(q);
+		if (SubNode *c = find(q->v ^ 1)) orphan_inv += c->in_tree;
+		const uint32_t d = (uint32_t)q->xnd;
+		for (uint32_t i = g.first(q->v), e = g.first(q->v) + g.n(q->v); i < e; ++i) {
+			const GfaArc &a = g.arc[i];
+			const uint32_t dt = d + (uint32_t)g.seg[a.w >> 1].len;
+			if (SubNode *c = find(a.w ^ 1))
+				if (!c->in_tree && !c->forced) continue;  // the other strand already left: a bidirected step
+			SubNode *p = find(a.w);
+			if (!p) {
+				p = gen(a.w, dt);
+			} else {
+				if (!p->in_tree) continue;  // a cycle back to an emitted vertex
+				tree.erase({p->xnd, p->v});
+				if (dt < (uint32_t)p->xnd) p->xnd = p->xnd >> 32 << 32 | dt;
+			}
+			if (tn_n(p) > 0) p->xnd -= 1ull << 32;
+			tree.insert({p->xnd, p->v});
+		}
+	}
+	order.clear();
+	for (size_t j = 0; j < L.size(); ++j) L[j]->pos = (uint32_t)j, order.push_back(L[j]->v);
+	sub_arc.clear();
+	sub_off.assign(L.size() + 1, 0);
+	for (size_t j = 0; j < L.size(); ++j) {
+		const size_t o0 = sub_arc.size();
+		for (uint32_t i = g.first(L[j]->v), e = g.first(L[j]->v) + g.n(L[j]->v); i < e; ++i)
+			if (SubNode *p = find(g.arc[i].w)) sub_arc.push_back((uint64_t)p->pos << 32 | i);
+		std::sort(sub_arc.begin() + o0, sub_arc.end());
+		sub_off[j + 1] = (uint32_t)sub_arc.size();
+	}
+}
+
+}  // namespace
+
 int gwf_gfa_load(const char *path, const char *start, gwf_graph_t **out) {
 	if (!path || !out) return GWF_ERR_ARG;
 	*out = nullptr;
 	LineReader lr(path);
 	if (!lr.ok()) return GWF_ERR_ARG;
-	std::vector<GfaSeg> segs;
-	std::unordered_map<std::string, uint32_t> id;
-	std::vector<GfaArc> arcs;
-	auto seg_id = [&](const std::string &nm) {
-		auto it = id.find(nm);
-		if (it != id.end()) return it->second;
-		uint32_t i = (uint32_t)segs.size();
-		segs.push_back(GfaSeg());
-		segs.back().name = nm;
-		id.emplace(nm, i);
-		return i;
-	};
+	GfaGraph g;
 	std::string line;
 	std::vector<std::string> f;
-	int64_t fa_cur = -1;
-	while (lr.next(line)) {
-		if (!line.empty() && line[0] == '>') {  // FASTA record = one segment
-			size_t e = line.find_first_of(" \t", 1);
-			fa_cur = seg_id(line.substr(1, e == std::string::npos ? std::string::npos : e - 1));
-			segs[fa_cur].has_s = true;
-			segs[fa_cur].seq.clear();
-			continue;
-		}
-		if (fa_cur >= 0 && !(line.size() >= 3 && line[1] == '\t')) {
-			segs[fa_cur].seq += line;
-			continue;
+	int64_t fa_cur = -1;  // segment of the FASTA record being read
+	std::string fa_seq;
+	auto fa_flush = [&]() {
+		if (fa_cur >= 0) {
+			g.seg[fa_cur].seq = fa_seq;
+			g.seg[fa_cur].has_seq = true;
+			g.seg[fa_cur].len = (int64_t)fa_seq.size();
 		}
 		fa_cur = -1;
-		if (line.size() < 3 || line[1] != '\t') continue;
-		if (line[0] == 'S') {
-			split_tabs(line, f);
-			if (f.size() < 3) return GWF_ERR_ARG;
-			uint32_t s = seg_id(f[1]);
-			segs[s].has_s = true;
-			segs[s].seq = f[2] == "*" ? std::string() : f[2];
-			for (size_t i = 3; i < f.size(); ++i)
-				if (f[i].compare(0, 5, "LN:i:") == 0) segs[s].ln = atoll(f[i].c_str() + 5);
-			if (f[2] == "*" && segs[s].ln > 0) segs[s].seq.assign((size_t)segs[s].ln, 'N');
-		} else if (line[0] == 'L') {
-			split_tabs(line, f);
-			if (f.size() < 5) return GWF_ERR_ARG;
-			if ((f[2] != "+" && f[2] != "-") || (f[4] != "+" && f[4] != "-")) return GWF_ERR_ARG;
-			GfaArc a;
-			a.v = seg_id(f[1]) << 1 | (f[2] == "-");
-			a.w = seg_id(f[3]) << 1 | (f[4] == "-");
-			if (!parse_overlap(f.size() > 5 ? f[5].c_str() : "", a.ov, a.ow)) return GWF_ERR_ARG;
-			arcs.push_back(a);
-		}
-	}
-	if (segs.empty()) return GWF_ERR_ARG;
-	// complement arcs (gfa_fix_symm_add) and duplicate removal
-	const size_t na0 = arcs.size();
-	for (size_t i = 0; i < na0; ++i) {
-		GfaArc c;
-		c.v = arcs[i].w ^ 1, c.w = arcs[i].v ^ 1, c.ov = arcs[i].ow, c.ow = arcs[i].ov;
-		arcs.push_back(c);
-	}
-	std::stable_sort(arcs.begin(), arcs.end(), [](const GfaArc &x, const GfaArc &y) {
-		return x.v != y.v ? x.v < y.v : x.w < y.w;
-	});
-	arcs.erase(std::unique(arcs.begin(), arcs.end(), [](const GfaArc &x, const GfaArc &y) {
-		return x.v == y.v && x.w == y.w;
-	}), arcs.end());
-	// drop links touching segments that never had an S line
-	arcs.erase(std::remove_if(arcs.begin(), arcs.end(), [&](const GfaArc &x) {
-		return !segs[x.v >> 1].has_s || !segs[x.w >> 1].has_s;
-	}), arcs.end());
-	const uint32_t nov = (uint32_t)segs.size() * 2;
-	std::vector<uint32_t> aoff(nov + 1, 0);
-	for (auto &a : arcs) ++aoff[a.v + 1];
-	for (uint32_t v = 0; v < nov; ++v) aoff[v + 1] += aoff[v];  // arcs are sorted by v
-	uint32_t v0;
-	if (start) {
-		auto it = id.find(start);
-		if (it == id.end() || !segs[it->second].has_s) return GWF_ERR_ARG;
-		v0 = it->second << 1;
-	} else {
-		uint32_t s = 0;
-		while (s < segs.size() && !segs[s].has_s) ++s;
-		if (s == segs.size()) return GWF_ERR_ARG;
-		v0 = s << 1;
-	}
-	// reachable oriented vertices from v0, in discovery order
-	std::vector<int64_t> idx(nov, -1);
-	std::vector<uint32_t> reach;
-	std::deque<uint32_t> bfs{v0};
-	idx[v0] = 0, reach.push_back(v0);
-	while (!bfs.empty()) {
-		uint32_t v = bfs.front();
-		bfs.pop_front();
-		for (uint32_t j = aoff[v]; j < aoff[v + 1]; ++j)
-			if (idx[arcs[j].w] < 0) {
-				idx[arcs[j].w] = (int64_t)reach.size();
-				reach.push_back(arcs[j].w);
-				bfs.push_back(arcs[j].w);
-			}
-	}
-	// Kahn order over the reachable subgraph, v0 forced first; on a cycle, the
-	// earliest-discovered remaining vertex is taken next.
-	const size_t nr = reach.size();
-	std::vector<uint32_t> indeg(nr, 0), order;
-	std::vector<char> done(nr, 0);
-	for (uint32_t v : reach)
-		for (uint32_t j = aoff[v]; j < aoff[v + 1]; ++j) ++indeg[idx[arcs[j].w]];
-	std::deque<uint32_t> ready;
-	size_t scan = 0;
-	auto emit = [&](uint32_t ri) {
-		done[ri] = 1;
-		order.push_back(ri);
-		uint32_t v = reach[ri];
-		for (uint32_t j = aoff[v]; j < aoff[v + 1]; ++j) {
-			uint32_t wi = (uint32_t)idx[arcs[j].w];
-			if (!done[wi] && --indeg[wi] == 0) ready.push_back(wi);
-		}
+		fa_seq.clear();
 	};
-	emit(0);
-	while (order.size() < nr) {
-		if (ready.empty()) {
-			while (done[scan]) ++scan;
-			emit((uint32_t)scan);
+	while (lr.next(line)) {
+		if (!line.empty() && line[0] == '>') {  // a FASTA record is a segment named s<index + 1>
+			fa_flush();                           // (gfa_parse_fa_hdr, gfa-io.c:326-340)
+			fa_cur = g.add_seg("s" + std::to_string(g.seg.size() + 1));
 			continue;
 		}
-		uint32_t ri = ready.front();
-		ready.pop_front();
-		if (!done[ri]) emit(ri);
+		if (fa_cur >= 0) {
+			if (!(line.size() >= 3 && line[1] == '\t')) {
+				fa_seq += line;
+				continue;
+			}
+			fa_flush();
+		}
+		if (line.size() < 3 || line[1] != '\t') continue;
+		split_tabs(line, f);
+		if (line[0] == 'S') {  // gfa_parse_S
+			if (f.size() < 3) continue;
+			const uint32_t sid = g.add_seg(f[1]);
+			GfaSeg &s = g.seg[sid];
+			int64_t LN = -1;
+			for (size_t i = 3; i < f.size(); ++i)
+				if (f[i].compare(0, 5, "LN:i:") == 0) LN = atoll(f[i].c_str() + 5);
+			s.has_seq = f[2] != "*";
+			s.seq = s.has_seq ? f[2] : std::string();
+			s.len = s.has_seq ? (int64_t)s.seq.size() : (LN >= 0 ? LN : 0);
+		} else if (line[0] == 'L') {  // gfa_parse_L
+			if (f.size() < 5) continue;
+			if ((f[2] != "+" && f[2] != "-") || (f[4] != "+" && f[4] != "-")) continue;
+			GfaArc a;
+			parse_overlap(f.size() > 5 ? f[5].c_str() : "", a.ov, a.ow);
+			a.v = g.add_seg(f[1]) << 1 | (f[2] == "-");
+			a.w = g.add_seg(f[3]) << 1 | (f[4] == "-");
+			a.link = (uint32_t)g.arc.size();
+			for (size_t i = 6; i < f.size(); ++i) {  // L1/L2: lengths implied by the link
+				if (f[i].compare(0, 5, "L1:i:") == 0 && a.ov != kNoOvl)
+					g.seg[a.v >> 1].len = std::max<int64_t>(g.seg[a.v >> 1].len, a.ov + atoll(f[i].c_str() + 5));
+				if (f[i].compare(0, 5, "L2:i:") == 0 && a.ow != kNoOvl)
+					g.seg[a.w >> 1].len = std::max<int64_t>(g.seg[a.w >> 1].len, a.ow + atoll(f[i].c_str() + 5));
+			}
+			g.arc.push_back(a);
+		}
 	}
-	std::vector<uint32_t> newid(nr);
-	for (size_t i = 0; i < nr; ++i) newid[order[i]] = (uint32_t)i;
+	fa_flush();
+	if (g.seg.empty()) return GWF_ERR_ARG;
+	g.finalize();
+	uint32_t v0 = 0;  // the first segment, forward strand (main.cu:368)
+	if (start) {
+		auto it = g.id.find(start);
+		if (it == g.id.end()) return GWF_ERR_ARG;
+		v0 = it->second << 1;
+	}
+	std::vector<uint32_t> order, sub_off;
+	std::vector<uint64_t> sub_arc;
+	extract_sub(g, v0, order, sub_arc, sub_off);
 	Builder b;
 	const unsigned char *ct = comp_table();
-	for (size_t i = 0; i < nr; ++i) {
-		uint32_t v = reach[order[i]];
-		const std::string &s = segs[v >> 1].seq;
+	for (uint32_t v : order) {  // gwf_gfa2gwf
+		const GfaSeg &sg = g.seg[v >> 1];
+		// a segment given as "*" has no bases; N stands for them (the
+		// reference would dereference a null sequence here)
+		std::string s = sg.has_seq ? sg.seq : std::string((size_t)sg.len, 'N');
+		s.resize((size_t)sg.len, 'N');
 		if (v & 1) {
 			std::string rc(s.size(), 'N');
 			for (size_t j = 0; j < s.size(); ++j) rc[j] = (char)ct[(unsigned char)s[s.size() - 1 - j]];
 			b.add(rc, v);
 		} else b.add(s, v);
 	}
-	for (size_t i = 0; i < nr; ++i) {
-		uint32_t v = reach[order[i]];
-		for (uint32_t j = aoff[v]; j < aoff[v + 1]; ++j)
-			b.link((uint32_t)i, newid[idx[arcs[j].w]], arcs[j].ow);
-	}
+	for (size_t i = 0; i < order.size(); ++i)
+		for (uint32_t k = sub_off[i]; k < sub_off[i + 1]; ++k)
+			b.link((uint32_t)i, (uint32_t)(sub_arc[k] >> 32), g.arc[(uint32_t)sub_arc[k]].ow);
 	*out = b.finish();
 	return *out ? GWF_OK : GWF_ERR_NOMEM;
 }
