@@ -30,6 +30,7 @@ struct DGraph {
 	const uint8_t *seq;
 	const uint4 *arc;
 	u32 n_vtx;
+	u32 total_len;  // sum of vertex lengths
 };
 
 struct DReads {
@@ -38,6 +39,7 @@ struct DReads {
 	const i32 *ql;
 	const i32 *v0;
 	i32 n;
+	i32 v1;               // end vertex of every read (-1: end anywhere)
 };
 
 // Element capacities of one read's working arrays (in HBM or in LDS).

@@ -517,7 +517,7 @@ enum { STEP_NEXT = 0, STEP_FOUND = 1, STEP_EMPTY = 2, STEP_SPILL = 3 };
 // idempotent).  HBM selects how sorts are staged.
 template <int NW, bool HBM>
 __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 ql, const Bufs &w, const Blk<NW> &b,
-                                        u64 *vis, i32 &n_a, i32 &n_h, i32 &hs, int64_t &chars) {
+                                        u64 *vis, i32 v1, i32 &n_a, i32 &n_h, i32 &hs, int64_t &chars) {
 	constexpr i32 NT = Blk<NW>::NT;
 	const i32 tid = (i32)b.tid();
 	const Caps &cap = w.c;
@@ -538,8 +538,12 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 		k = extend_wave(act, g.seq, vi.x, q, d, k, max_k, &chars);
 		if (act) w.P[j] = mk(v, d, k);
 		const bool qend = act && d + k == ql - 1;
-		found |= qend;
-		const bool vend = act && !qend && k == vl - 1;
+		// the end (gwf-ed.cu:1231-1236): anywhere, or at the last base of v1
+		const bool fnd = qend && (v1 < 0 || (v == (u32)v1 && k == vl - 1));
+		found |= fnd;
+		// vertex ends go to the closure; with v1 >= 0 that includes query
+		// ends, which continue by deletion into the next vertices
+		const bool vend = act && !fnd && k == vl - 1;
 		const i32 sl = b.slot(vend, &n_f);
 		if (vend && sl < cap.f) w.F0[sl] = mk(v, d, k);
 	}
@@ -610,6 +614,9 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 			const i32 i = d + k;
 			const uint4 vi = act ? g.vinfo[v] : make_uint4(0, 0, 0, 0);
 			const bool rec = act && closure_origin;
+			// query end at a vertex end (only with v1 >= 0): delete into every
+			// next vertex, nothing else (gwf-ed.cu:1244-1254)
+			const bool qe = act && i + 1 == ql;
 			const uint8_t c = act ? q[i + 1] : 0;
 			const i32 deg = act ? (i32)vi.w : 0;
 			const i32 maxdeg = b.max(deg);
@@ -619,22 +626,30 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 				const uint4 ar = has ? g.arc[vi.z + e] : make_uint4(0, 0, 0, 0);
 				const u32 wv = ar.x;
 				const i32 ol = (i32)(ar.y & 0xFFFFFFu);
-				const bool m = has && (uint8_t)(ar.y >> 24) == c;
+				const bool m = has && !qe && (uint8_t)(ar.y >> 24) == c;
 				n_ext += m;
-				const bool absent = has && vis_insert(vis, (u64)wv << 32 | (u32)(i + 1));
+				const bool absent = has && !qe && vis_insert(vis, (u64)wv << 32 | (u32)(i + 1));
 				const bool pa = absent && m;
 				const i32 sa = b.slot(pa, &n_na);
 				if (pa && sa < cap.f) w.F1[sa] = mk(wv, i + 1 - ol, ol);
 				const bool px = absent && !m;
+				const bool pq = has && qe;
 				i32 tx;
-				const i32 sx = n_x + b.scan(px ? 2 : 0, &tx);
+				const i32 sx = n_x + b.scan(px ? 2 : pq ? 1 : 0, &tx);
 				n_x += tx;
 				if (px && sx + 1 < cap.x) {
 					w.X[sx] = mk(wv, i - ol, ol);
 					w.X[sx + 1] = mk(wv, i + 1 - ol, ol);
 				}
+				if (pq && sx < cap.x) w.X[sx] = mk(wv, i - ol, ol);
 			}
-			const bool pi = rec && (deg == 0 || n_ext != deg);
+			const bool pi = rec && !qe && (deg == 0 || n_ext != deg);
+			// with v1 >= 0 the exits of states that entered their vertex in
+			// this step join the history too (gwf-ed.cu:1177-1183); with
+			// v1 < 0 they change no distance and are left out
+			const bool t2 = rec && !qe && v1 >= 0;
+			const i32 st2 = b.slot(t2, &n_t);
+			if (t2 && st2 < cap.t) w.T[st2] = diag_id(v, d);
 			const i32 si = b.slot(pi, &n_x);
 			if (pi && si < cap.x) w.X[si] = mk(v, d + 1, k);
 		}
@@ -654,19 +669,22 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 			const i32 max_k = (ql - d < vl ? ql - d : vl) - 1;
 			k = extend_wave(act, g.seq, vi.x, q, d, k, max_k, &chars);
 			const bool qend = act && d + k == ql - 1;
-			found |= qend;
-			const bool vend = act && !qend && k == vl - 1;
+			const bool fnd = qend && (v1 < 0 || (v == (u32)v1 && k == vl - 1));
+			found |= fnd;
+			const bool vend = act && !fnd && k == vl - 1;
 			const bool mid = act && !qend && !vend;
+			const bool qmid = act && qend && !fnd && k + 1 < vl;  // query end mid-vertex: deletion only (gwf-ed.cu:1237-1241)
 			const i32 sf = b.slot(vend, &n_f2);
 			if (vend && sf < cap.f) w.F0[sf] = mk(v, d, k);
 			i32 tx;
-			const i32 sx = n_x + b.scan(mid ? 3 : 0, &tx);
+			const i32 sx = n_x + b.scan(mid ? 3 : qmid ? 1 : 0, &tx);
 			n_x += tx;
 			if (mid && sx + 2 < cap.x) {
 				w.X[sx] = mk(v, d - 1, k + 1);
 				w.X[sx + 1] = mk(v, d, k + 1);
 				w.X[sx + 2] = mk(v, d + 1, k);
 			}
+			if (qmid && sx < cap.x) w.X[sx] = mk(v, d - 1, k + 1);
 		}
 		if (b.any(found)) return STEP_FOUND;
 		if (n_f2 > cap.f || n_x > cap.x) return STEP_SPILL;
@@ -676,21 +694,52 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 	}
 
 	// ---- capacity checks for the destructive part of the step (P is free from here on)
-	if (n_s + n_x > cap.m || n_x > cap.m || 4 * n_t1 + 2 * n_h > cap.m || n_h + n_t1 > cap.h) return STEP_SPILL;
+	if (n_s + n_x > cap.m || n_x > cap.m || n_t > cap.t || n_t > cap.m || 4 * n_t + 2 * n_h > cap.m ||
+	    n_h + n_t > cap.h)
+		return STEP_SPILL;
+
+	// ---- exits recorded by the closure (v1 >= 0 only): sort T, drop repeats
+	i32 n_tt = n_t1;
+	if (n_t > n_t1) {
+		const u64 *S;
+		if constexpr (HBM) {
+			sort_hbm(w.T, n_t, w.sort_lds, w.sort_cap, w.sort_tmp, b);
+			S = w.T;
+		} else {
+			sort_runs(w.T, n_t, w.P, w.mw, b);
+			S = w.P;
+		}
+		u64 *D = HBM ? w.P : w.T;
+		i32 nu = 0;
+		for (i32 base = 0; base < n_t; base += NT) {
+			const i32 j = base + tid;
+			const bool act = j < n_t;
+			const u64 x = act ? S[j] : 0;
+			const bool keep = act && (j == 0 || S[j - 1] != x);
+			const i32 sl = b.slot(keep, &nu);
+			if (keep) D[sl] = x;
+		}
+		b.sync();
+		if constexpr (HBM) {
+			for (i32 j = tid; j < nu; j += NT) w.T[j] = D[j];
+			b.sync();
+		}
+		n_tt = nu;
+	}
 
 	// ---- P5: history H[hs ^ 1] = coalesce(H[hs] u runs of this step's exits)
 	u64 *Hcur_lo = hs ? w.H1lo : w.H0lo, *Hcur_hi = hs ? w.H1hi : w.H0hi;
 	u64 *Hlo = hs ? w.H0lo : w.H1lo, *Hhi = hs ? w.H0hi : w.H1hi;
 	i32 nh2;
 	{
-		u64 *TIlo = w.P, *TIhi = w.P + n_t1;
+		u64 *TIlo = w.P, *TIhi = w.P + n_tt;
 		i32 n_ti = 0;
-		for (i32 base = 0; base < n_t1; base += NT) {
+		for (i32 base = 0; base < n_tt; base += NT) {
 			const i32 j = base + tid;
-			const bool act = j < n_t1;
+			const bool act = j < n_tt;
 			const u64 x = act ? w.T[j] : 0;
 			const bool start = act && (j == 0 || w.T[j - 1] + 1 != x);
-			const bool end = act && (j + 1 == n_t1 || w.T[j + 1] != x + 1);
+			const bool end = act && (j + 1 == n_tt || w.T[j + 1] != x + 1);
 			i32 cnt;
 			const i32 e = b.scan(start ? 1 : 0, &cnt);
 			if (start) TIlo[n_ti + e] = x;
@@ -698,7 +747,7 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 			n_ti += cnt;
 		}
 		b.sync();
-		u64 *Clo = w.P + 2 * n_t1, *Chi = Clo + (n_h + n_ti);
+		u64 *Clo = w.P + 2 * n_tt, *Chi = Clo + (n_h + n_ti);
 		merge_intervals2(Hcur_lo, Hcur_hi, n_h, TIlo, TIhi, n_ti, Clo, Chi, b);
 		nh2 = coalesce(Clo, Chi, n_h + n_ti, Hlo, Hhi, b);
 	}
@@ -784,8 +833,8 @@ __device__ __forceinline__ void beat_put(GwfBeat *bt, i32 rid, i32 s, i32 n_a, i
 // Align one read: LDS-resident while it fits (USE_LDS), then in HBM.  beat
 // (may be null) receives a progress record every GWF_BEAT_EVERY steps.
 template <int NW, bool USE_LDS>
-__device__ __forceinline__ ReadOut align_read(const DGraph &g, const uint8_t *q, i32 ql, u32 v0, const Bufs &lds, const Bufs &hbm,
-                              const Blk<NW> &b, u64 *vis, GwfBeat *beat, i32 rid) {
+__device__ __forceinline__ ReadOut align_read(const DGraph &g, const uint8_t *q, i32 ql, u32 v0, i32 v1, const Bufs &lds,
+                              const Bufs &hbm, const Blk<NW> &b, u64 *vis, GwfBeat *beat, i32 rid) {
 	ReadOut o;
 	o.ed = -1, o.status = GWF_ST_OK, o.steps = 0, o.states = 0, o.chars = 0, o.spill_step = -1;
 	i32 n_a = 1, n_h = 0, hs = 0;
@@ -796,12 +845,13 @@ __device__ __forceinline__ ReadOut align_read(const DGraph &g, const uint8_t *q,
 	for (i32 s = 0;; ++s) {
 		o.steps += 1;
 		o.states += n_a;
-		if (s > ql + 1) { o.status = GWF_ST_LIMIT; break; }
+		// the ED is at most ql (v1 < 0), or ql plus the bases deleted on the way to v1
+		if (s > ql + 1 + (v1 >= 0 ? (int64_t)g.total_len : 0)) { o.status = GWF_ST_LIMIT; break; }
 		if (beat && (s % GWF_BEAT_EVERY) == 0 && b.tid() == 0) beat_put(beat, rid, s, n_a, !in_lds);
 		int rc = STEP_SPILL;
 		if constexpr (USE_LDS) {
 			if (in_lds) {
-				rc = gwf_step<NW, false>(g, q, ql, lds, b, vis, n_a, n_h, hs, chars);
+				rc = gwf_step<NW, false>(g, q, ql, lds, b, vis, v1, n_a, n_h, hs, chars);
 				if (rc == STEP_SPILL) {  // the step input is intact: move it to HBM and redo the step there
 					spill_state(lds, hbm, n_a, n_h, hs, b);
 					in_lds = false;
@@ -809,7 +859,7 @@ __device__ __forceinline__ ReadOut align_read(const DGraph &g, const uint8_t *q,
 				}
 			}
 		}
-		if (rc == STEP_SPILL) rc = gwf_step<NW, true>(g, q, ql, hbm, b, vis, n_a, n_h, hs, chars);
+		if (rc == STEP_SPILL) rc = gwf_step<NW, true>(g, q, ql, hbm, b, vis, v1, n_a, n_h, hs, chars);
 		if (rc == STEP_SPILL) { o.status = GWF_ST_OVERFLOW; break; }
 		if (rc == STEP_FOUND) { o.ed = s; break; }
 		if (rc == STEP_EMPTY) { o.ed = -1; break; }
@@ -861,7 +911,7 @@ __global__ __launch_bounds__(WAVE) void gwf_plain_kernel(DGraph g, DReads rd, co
 	w.sort_lds = lds_sort;
 	w.sort_cap = SORT_CHUNK;
 	const u64 t0 = wall_clock64();
-	const ReadOut o = align_read<1, false>(g, rd.seq + rd.off[r], rd.ql[r], (u32)rd.v0[r], w, w, b, vis, nullptr, r);
+	const ReadOut o = align_read<1, false>(g, rd.seq + rd.off[r], rd.ql[r], (u32)rd.v0[r], rd.v1, w, w, b, vis, nullptr, r);
 	write_out<1>(o, r, t0, out_ed, out_status, out_counts, b);
 }
 
@@ -911,7 +961,7 @@ __global__ __launch_bounds__(WAVE *NW) void gwf_persistent_kernel(DGraph g, DRea
 		if (qi >= q_n) break;
 		const i32 r = order ? order[q_lo + qi] : q_lo + qi;
 		const u64 t0 = wall_clock64();
-		const ReadOut o = align_read<NW, true>(g, rd.seq + rd.off[r], rd.ql[r], (u32)rd.v0[r], lb, hb, b, vis, beat, r);
+		const ReadOut o = align_read<NW, true>(g, rd.seq + rd.off[r], rd.ql[r], (u32)rd.v0[r], rd.v1, lb, hb, b, vis, beat, r);
 		write_out<NW>(o, r, t0, out_ed, out_status, out_counts, b);
 		b.sync();
 	}
@@ -1019,7 +1069,7 @@ __global__ __launch_bounds__(WAVE *NW) void gwf_serve_kernel(SrvArgs a) {
 		hb.sort_lds = arena;
 		hb.sort_cap = sort_cap;
 		const u64 t0 = wall_clock64();
-		const ReadOut o = align_read<NW, true>(a.g, dq, e.ql, (u32)e.v0, lb, hb, b, vis, beat, (i32)seq);
+		const ReadOut o = align_read<NW, true>(a.g, dq, e.ql, (u32)e.v0, -1, lb, hb, b, vis, beat, (i32)seq);
 		int64_t c = o.chars;
 #pragma unroll
 		for (int s = 32; s > 0; s >>= 1) c += __shfl_xor(c, s);

@@ -389,6 +389,7 @@ int gwf_dgraph_upload(const gwf_graph_t *g, gwf_dgraph_t **out) {
 	dg->g.arc = (const uint4 *)p;
 	if (e == hipSuccess) e = hipMemcpy(p, arcs.data(), b_arc, hipMemcpyHostToDevice);
 	dg->g.n_vtx = nv;
+	dg->g.total_len = (u32)total;
 	dg->n_vtx = nv;
 	if (e != hipSuccess) {
 		set_err("gwf_dgraph_upload: hipMemcpy failed: %s", hipGetErrorString(e));
@@ -457,6 +458,7 @@ int gwf_dreads_upload(int32_t n, const char *seqs, const int64_t *off, const int
 	}
 	dr->ql_probe = (const i32 *)p;
 	dr->r.n = n;
+	dr->r.v1 = -1;
 	if (e != hipSuccess) {
 		set_err("gwf_dreads_upload: hipMemcpy failed: %s", hipGetErrorString(e));
 		gwf_dreads_free(dr);
@@ -609,6 +611,7 @@ struct BinnedPlan {
 	PersistCfg bin[GWF_MAX_BINS];
 	BinBounds bb;
 	i32 n = 0;
+	i32 v1 = -1;
 	bool concurrent = false;
 	GwfBeat *beat = nullptr;  // heartbeat records: beat_stride per bin (probe: bin n_bins)
 	i32 beat_stride = 0;
@@ -620,8 +623,10 @@ hipError_t enqueue_binned(RunCtx &cx, const gwf_dgraph_t *dg, const gwf_dreads_t
 	do {                                  \
 		if ((e = (x)) != hipSuccess) return e; \
 	} while (0)
-	DReads rp = dr->r;
+	DReads rp = dr->r, rr = dr->r;
 	rp.ql = dr->ql_probe;
+	rp.v1 = -1;  // the probe only estimates the error rate
+	rr.v1 = p.v1;
 	ENQ(gwf_launch_fill(cx.boff + 16, 1, 0, cx.s));
 	ENQ(gwf_launch_fill(cx.boff + 17, 1, p.n, cx.s));
 	ENQ(gwf_launch_fill(cx.head + 16, 1, 0, cx.s));
@@ -636,7 +641,7 @@ hipError_t enqueue_binned(RunCtx &cx, const gwf_dgraph_t *dg, const gwf_dreads_t
 	for (int b = p.bb.n_bins - 1; b >= 0; --b) {  // costliest bin first
 		hipStream_t st = p.concurrent ? cx.sb[b] : cx.s;
 		u64 *scr = p.concurrent ? cx.scr[b] : cx.scr[0];
-		ENQ(gwf_launch_persistent(dg->g, dr->r, cx.ord, cx.boff, b, cx.head, p.bin[b], scr,
+		ENQ(gwf_launch_persistent(dg->g, rr, cx.ord, cx.boff, b, cx.head, p.bin[b], scr,
 		                          gwf_ws_bytes(p.bin[b].hbm), cx.ed, cx.status, cx.cnt,
 		                          p.beat ? p.beat + (size_t)b * p.beat_stride : nullptr, st));
 	}
@@ -650,13 +655,14 @@ hipError_t enqueue_binned(RunCtx &cx, const gwf_dgraph_t *dg, const gwf_dreads_t
 	return hipSuccess;
 }
 
-bool run_binned(RunCtx &cx, RunAcc &acc, const gwf_dgraph_t *dg, const gwf_dreads_t *dr, i32 maxql, size_t budget,
-                bool graph_ok) {
+bool run_binned(RunCtx &cx, RunAcc &acc, const gwf_dgraph_t *dg, const gwf_dreads_t *dr, i32 v1, i32 maxql,
+                size_t budget, bool graph_ok) {
 	const i32 n = dr->r.n;
 	u64 tv = 0;
 	const gwf_tune_t t = tune_current(&tv);
 	BinnedPlan p;
 	p.n = n;
+	p.v1 = v1;
 	p.probe = make_cfg(cx, 1, 256, 8, std::min(maxql, kProbeLen));
 	p.bb.n_bins = t.n_bins;
 	size_t ws_sum = 0, ws_max = 0;
@@ -711,7 +717,7 @@ bool run_binned(RunCtx &cx, RunAcc &acc, const gwf_dgraph_t *dg, const gwf_dread
 		return true;
 	}
 	// everything the captured launches point at or were sized by
-	std::vector<uintptr_t> key = {(uintptr_t)dg, (uintptr_t)dr, (uintptr_t)n, (uintptr_t)tv, (uintptr_t)maxql,
+	std::vector<uintptr_t> key = {(uintptr_t)dg, (uintptr_t)dr, (uintptr_t)n, (uintptr_t)tv, (uintptr_t)maxql, (uintptr_t)(v1 + 1),
 	                              (uintptr_t)p.concurrent, (uintptr_t)cx.ed, (uintptr_t)cx.status, (uintptr_t)cx.cnt,
 	                              (uintptr_t)cx.scratch, (uintptr_t)cx.pe, (uintptr_t)cx.ps, (uintptr_t)cx.key,
 	                              (uintptr_t)cx.ord};
@@ -753,7 +759,7 @@ int gwf_ed_run_timed(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t v1, uint32_t ma
 	(void)max_lag;  // accepted and ignored, as in the reference (gwf-ed.cu:1286-1287)
 	const auto t0 = std::chrono::steady_clock::now();
 	if (!dg || !dr || (!out && dr->r.n > 0)) { set_err("gwf_ed_run: NULL argument"); return GWF_ERR_ARG; }
-	if (v1 >= 0) { set_err("gwf_ed_run: v1 >= 0 is not supported on the GPU path"); return GWF_ERR_ARG; }
+	if (v1 >= 0 && (u32)v1 >= dg->n_vtx) { set_err("gwf_ed_run: v1 %d >= n_vtx %u", v1, dg->n_vtx); return GWF_ERR_ARG; }
 	if (kernel != GWF_KERNEL_PLAIN && kernel != GWF_KERNEL_PERSISTENT) {
 		set_err("gwf_ed_run: unknown kernel %d", kernel);
 		return GWF_ERR_ARG;
@@ -781,7 +787,7 @@ int gwf_ed_run_timed(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t v1, uint32_t ma
 	if (kernel == GWF_KERNEL_PERSISTENT) {
 		i32 maxql = 0;
 		for (i32 q : dr->ql) maxql = std::max(maxql, q);
-		ok = run_binned(cx, acc, dg, dr, maxql, budget, true);
+		ok = run_binned(cx, acc, dg, dr, v1, maxql, budget, true);
 		if (ok) ok = hipMemcpy(status.data(), cx.status, 4 * (size_t)n, hipMemcpyDeviceToHost) == hipSuccess;
 		for (i32 i = 0; i < n && ok; ++i)
 			if (status[i] == GWF_ST_OVERFLOW) todo.push_back(i);
@@ -796,7 +802,9 @@ int gwf_ed_run_timed(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t v1, uint32_t ma
 	}
 	// plain kernel (and re-runs of reads that overflowed their HBM workspace)
 	while (ok && rc == GWF_OK && !todo.empty()) {
-		ok = run_plain(cx, acc, dg, dr->r, dr->ql, todo, scale, budget);
+		DReads rv = dr->r;
+		rv.v1 = v1;
+		ok = run_plain(cx, acc, dg, rv, dr->ql, todo, scale, budget);
 		if (!ok) break;
 		ok = hipMemcpy(status.data(), cx.status, 4 * (size_t)n, hipMemcpyDeviceToHost) == hipSuccess;
 		std::vector<i32> again;
@@ -898,7 +906,7 @@ int gwf_autotune(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t max_evals, gwf_tune
 				g_tune_version += 1;
 			}
 			const double before = acc.st.kernel_ms;
-			if (!run_binned(cx, acc, dg, dr, maxql, budget, false)) return -1.0;
+			if (!run_binned(cx, acc, dg, dr, -1, maxql, budget, false)) return -1.0;
 			if (hipMemcpy(status.data(), cx.status, 4 * (size_t)n, hipMemcpyDeviceToHost) != hipSuccess) return -1.0;
 			i32 over = 0;
 			for (i32 x : status) over += x != GWF_ST_OK;

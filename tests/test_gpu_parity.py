@@ -208,3 +208,49 @@ def test_mixed_overlaps_rejected(gpu):
     g = gpu.Graph.from_lists(["ACGT", "CCGT", "GATTACA"], [(0, 2, 0), (1, 2, 2), (0, 1, 0)])
     with pytest.raises(gpu.GwfError):
         gpu.DeviceGraph(g)
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("seed,dag,ovl", [(21, True, False), (22, False, False), (23, True, True), (24, False, True)])
+def test_end_vertex_v1_vs_oracle(gpu, orc, seed, dag, ovl, kernel):
+    """v1 >= 0 (end at the last base of v1, gwf-ed.cu:1231-1254): the GPU must
+    reproduce the reference algorithm's result, which is not always the exact
+    distance (the oracle is the reference restated, not the DP)."""
+    _say(f"case: end vertex v1, random graphs seed={seed} dag={dag} ovl={ovl} kernel {kernel}")
+    rng = random.Random(seed)
+    for it in range(12):
+        g, seqs = _random_graph(gpu, rng, rng.randint(2, 30), dag, ovl)
+        arcs_of = {}
+        for v, w, o in g.arcs():
+            arcs_of.setdefault(v, []).append((w, o))
+        reads = [_sample(rng, seqs, arcs_of, rng.randint(0, 150), rng.choice([0.0, 0.05, 0.2]))
+                 for _ in range(12)]
+        r = gpu.Reads.from_list(reads)
+        g.index()
+        for v1 in sorted({0, g.n_vtx - 1, rng.randrange(g.n_vtx)}):
+            want, _, _ = orc.batch(g, r, v1=v1)
+            dg, dr = gpu.DeviceGraph(g), gpu.DeviceReads(r)
+            try:
+                got, _ = gpu.ed_run(dg, dr, kernel=kernel, v1=v1)
+            finally:
+                dr.free()
+                dg.free()
+            assert np.array_equal(got, want), f"iteration {it} v1={v1}: gpu {got.tolist()} oracle {want.tolist()}"
+
+
+def test_end_vertex_v1_config_graph(gpu, orc):
+    _say("case: end vertex v1 on the small benchmark graph")
+    g, r = gpu.make_workload(1, 40)
+    g.index()
+    dg, dr = gpu.DeviceGraph(g), gpu.DeviceReads(r)
+    try:
+        for v1 in (5, g.n_vtx // 3):
+            want, _, _ = orc.batch(g, r, v1=v1)
+            for kernel in KERNELS:
+                got, _ = gpu.ed_run(dg, dr, kernel=kernel, v1=v1)
+                assert np.array_equal(got, want), (v1, kernel)
+        q = r.read(0)
+        assert gpu.gwf_ed(g, q, v1=7) == orc.ed(g, q, v1=7)
+    finally:
+        dr.free()
+        dg.free()

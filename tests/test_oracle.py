@@ -204,3 +204,22 @@ def test_batch_runner_threads(gw, orc):
     b, st8, t = orc.batch(g, r, threads=4)
     assert a.tolist() == b.tolist()
     assert st1["states"] == st8["states"] and st1["steps"] == int(a.sum()) + len(a)
+
+
+def test_v1_skip_insertion_case(gw, orc):
+    """A case where the reference algorithm's end-at-v1 result is not the
+    exact distance: the read needs an insertion after the last base of v1
+    (vertex 13), but every arc out of v1 matches the next query base, so the
+    reference adds no insertion there (gwf-ed.cu:1225-1226) and ends later.
+    Recording the exits of vertices entered in the same step
+    (gwf-ed.cu:1177-1183) also matters here: without them the result is 5."""
+    seqs = ['G', 'ACCCACCCACCAAACAAACCCCCA', 'A', 'A', 'CCA', 'A', 'AGGCGACCAAGA', 'AAA', 'CC',
+            'ACCACCCCCCGCGAGG', 'GGGAGACAC', 'ACAAGGGCCGCAAGCGCGC', 'AA', 'C', 'CCA']
+    arcs = [(0, 2, 0), (0, 4, 1), (1, 4, 1), (1, 12, 1), (2, 9, 2), (3, 6, 9), (3, 13, 0), (5, 7, 1), (7, 12, 1),
+            (9, 13, 0), (10, 12, 1), (10, 13, 0), (13, 14, 2)]
+    g = gw.Graph.from_lists(seqs, arcs)
+    g.index()
+    q = "GACACCCCCCGCGAGGCA"
+    assert orc.dp(g, q, v1=13) == 1
+    assert orc.ed(g, q, v1=13) == 8
+    assert orc.ed(g, q[:-1], v1=13) == 0      # without the trailing base the end is exact
