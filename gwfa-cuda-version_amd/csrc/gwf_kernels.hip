@@ -472,8 +472,9 @@ __device__ __forceinline__ bool vis_insert(u64 *vis, u64 key) {
 struct Bufs {
 	u64 *P, *Q, *X, *F0, *F1, *T;
 	u64 *H0lo, *H0hi, *H1lo, *H1hi;  // forbidden-interval history, double buffered
-	u64 *sort_lds;    // HBM mode: LDS scratch for sorting runs
-	i32 sort_cap;     // its capacity (power of two)
+	u64 *sort_lds;    // HBM mode: LDS scratch for sorting runs and staging
+	i32 sort_cap;     // its capacity for bitonic runs (power of two)
+	i32 sort_n;       // its size in elements
 	i32 mw;           // LDS sort: runs of 64 merged in one rank pass when at most mw of them
 	u64 *sort_tmp;    // HBM mode: merge temp (max(x, t) elements)
 	Caps c;
@@ -503,6 +504,7 @@ __device__ inline Bufs carve(u64 *p, const Caps &c, bool with_tmp) {
 	w.H1hi = p; p += c.h;
 	w.sort_tmp = with_tmp ? p : nullptr;
 	w.sort_lds = nullptr;
+	w.sort_n = 0;
 	w.mw = 8;
 	w.sort_cap = 0;
 	return w;
@@ -754,49 +756,135 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 
 	// ---- P4: sort the closure successors into P
 	const u64 *Xs;
+	i32 lds_used = 0;  // HBM mode: elements of the LDS scratch holding the sorted X
 	if constexpr (HBM) {
-		sort_hbm(w.X, n_x, w.sort_lds, w.sort_cap, w.sort_tmp, b);
-		Xs = w.X;
+		if (2 * n_x <= w.sort_n) {  // stage X in LDS, sort it there and merge from there
+			for (i32 t = tid; t < n_x; t += NT) w.sort_lds[t] = w.X[t];
+			b.sync();
+			sort_runs(w.sort_lds, n_x, w.sort_lds + n_x, w.mw, b);
+			Xs = w.sort_lds + n_x;
+			lds_used = 2 * n_x;
+		} else {
+			sort_hbm(w.X, n_x, w.sort_lds, w.sort_cap, w.sort_tmp, b);
+			Xs = w.X;
+		}
 	} else {
 		sort_runs(w.X, n_x, w.P, w.mw, b);
 		Xs = w.P;
 	}
 
-	// ---- P6: merge the sorted X into Q in place (merge path, right to left; Q first on ties)
 	const i32 n_m = n_s + n_x;
-	if (n_x > 0) {
-		for (i32 hi = n_m; hi > 0; hi -= NT) {
-			const i32 p = hi - NT + tid;
-			u64 val = 0;
-			if (p >= 0) {
-				i32 lo = p - n_x > 0 ? p - n_x : 0, up = p < n_s ? p : n_s;
-				while (lo < up) {
-					const i32 mid = (lo + up) >> 1;
-					if (w.Q[mid] <= Xs[p - 1 - mid]) lo = mid + 1;
-					else up = mid;
-				}
-				const i32 i = lo, jx = p - lo;
-				val = (i < n_s && (jx >= n_x || w.Q[i] <= Xs[jx])) ? w.Q[i] : Xs[jx];
-			}
-			b.sync();
-			if (p >= 0) w.Q[p] = val;
-			b.sync();
-		}
-	}
-
-	// ---- P7: furthest state per diagonal, outside the forbidden intervals
 	i32 n_new = 0;
-	for (i32 base = 0; base < n_m; base += NT) {
-		const i32 j = base + tid;
-		const bool act = j < n_m;
-		const u64 x = act ? w.Q[j] : 0;
-		const u64 dg = key_diag(x);
-		const bool first = act && (j == 0 || key_diag(w.Q[j - 1]) != dg);
-		const bool keep = first && !forbidden(Hlo, Hhi, nh2, dg);
-		const i32 sl = b.slot(keep, &n_new);
-		if (keep) w.P[sl] = x;
+	if (HBM && n_m <= 2 * cap.f) {
+		// ---- P6+P7 fused (arrays in HBM): thread t takes the contiguous range
+		// [t E, t E + E) of the merged order.  One merge-path search gives its
+		// start in Q and X; it then merges sequentially (Q first on ties),
+		// keeps the first (furthest) state of each diagonal, and tests the
+		// history with a pointer that only moves forward (one search for the
+		// first state).  Outputs go to F0..F1 (EMPTY_KEY when dropped) and are
+		// compacted into P.  A handful of dependent loads per thread instead of
+		// a binary search in HBM per state.
+		u64 *M = w.F0;  // F0 and F1 are adjacent: 2 cap.f >= n_m
+		// the new history in LDS too, when it fits beside the sorted X
+		const u64 *Hl = Hlo, *Hh = Hhi;
+		if (lds_used + 2 * nh2 <= w.sort_n) {
+			u64 *hl = w.sort_lds + lds_used, *hh = hl + nh2;
+			for (i32 t = tid; t < nh2; t += NT) hl[t] = Hlo[t], hh[t] = Hhi[t];
+			b.sync();
+			Hl = hl, Hh = hh;
+		}
+		const i32 E = (n_m + NT - 1) / NT;
+		const i32 o0 = tid * E;
+		const i32 o1 = o0 + E < n_m ? o0 + E : n_m;
+		if (o0 < n_m) {
+			i32 lo = o0 - n_x > 0 ? o0 - n_x : 0, up = o0 < n_s ? o0 : n_s;
+			while (lo < up) {
+				const i32 mid = (lo + up) >> 1;
+				if (w.Q[mid] <= Xs[o0 - 1 - mid]) lo = mid + 1;
+				else up = mid;
+			}
+			i32 i = lo, jx = o0 - lo;
+			u64 prev_dg = ~0ull;  // diag of the merged state before o0 (none: o0 == 0)
+			if (o0 > 0) {
+				const bool from_q = i > 0 && (jx == 0 || w.Q[i - 1] > Xs[jx - 1]);
+				prev_dg = key_diag(from_q ? w.Q[i - 1] : Xs[jx - 1]);
+			}
+			u64 qh = i < n_s ? w.Q[i] : EMPTY_KEY, xh = jx < n_x ? Xs[jx] : EMPTY_KEY;
+			i32 h = -1;
+			for (i32 o = o0; o < o1; ++o) {
+				const bool takeq = i < n_s && (jx >= n_x || qh <= xh);
+				const u64 x = takeq ? qh : xh;
+				if (takeq) {
+					++i;
+					qh = i < n_s ? w.Q[i] : EMPTY_KEY;
+				} else {
+					++jx;
+					xh = jx < n_x ? Xs[jx] : EMPTY_KEY;
+				}
+				const u64 dg = key_diag(x);
+				bool keep = dg != prev_dg;
+				prev_dg = dg;
+				if (keep) {
+					if (h < 0) {  // first interval ending after dg
+						i32 a = 0, c = nh2;
+						while (a < c) {
+							const i32 mid = (a + c) >> 1;
+							if (Hh[mid] <= dg) a = mid + 1;
+							else c = mid;
+						}
+						h = a;
+					} else {
+						while (h < nh2 && Hh[h] <= dg) ++h;
+					}
+					keep = !(h < nh2 && Hl[h] <= dg);  // coalesced intervals: disjoint, sorted
+				}
+				M[o] = keep ? x : EMPTY_KEY;
+			}
+		}
+		b.sync();
+		for (i32 base = 0; base < n_m; base += NT) {
+			const i32 j = base + tid;
+			const u64 x = j < n_m ? M[j] : EMPTY_KEY;
+			const bool keep = x != EMPTY_KEY;
+			const i32 sl = b.slot(keep, &n_new);
+			if (keep) w.P[sl] = x;
+		}
+		b.sync();
+	} else {
+		// ---- P6: merge the sorted X into Q in place (merge path, right to left; Q first on ties)
+		if (n_x > 0) {
+			for (i32 hi = n_m; hi > 0; hi -= NT) {
+				const i32 p = hi - NT + tid;
+				u64 val = 0;
+				if (p >= 0) {
+					i32 lo = p - n_x > 0 ? p - n_x : 0, up = p < n_s ? p : n_s;
+					while (lo < up) {
+						const i32 mid = (lo + up) >> 1;
+						if (w.Q[mid] <= Xs[p - 1 - mid]) lo = mid + 1;
+						else up = mid;
+					}
+					const i32 i = lo, jx = p - lo;
+					val = (i < n_s && (jx >= n_x || w.Q[i] <= Xs[jx])) ? w.Q[i] : Xs[jx];
+				}
+				b.sync();
+				if (p >= 0) w.Q[p] = val;
+				b.sync();
+			}
+		}
+
+		// ---- P7: furthest state per diagonal, outside the forbidden intervals
+		for (i32 base = 0; base < n_m; base += NT) {
+			const i32 j = base + tid;
+			const bool act = j < n_m;
+			const u64 x = act ? w.Q[j] : 0;
+			const u64 dg = key_diag(x);
+			const bool first = act && (j == 0 || key_diag(w.Q[j - 1]) != dg);
+			const bool keep = first && !forbidden(Hlo, Hhi, nh2, dg);
+			const i32 sl = b.slot(keep, &n_new);
+			if (keep) w.P[sl] = x;
+		}
+		b.sync();
 	}
-	b.sync();
 	n_a = n_new;
 	n_h = nh2;
 	hs ^= 1;
@@ -910,6 +998,7 @@ __global__ __launch_bounds__(WAVE) void gwf_plain_kernel(DGraph g, DReads rd, co
 	Bufs w = carve(scratch + (size_t)blk * ws_elems_per_read, caps, true);
 	w.sort_lds = lds_sort;
 	w.sort_cap = SORT_CHUNK;
+	w.sort_n = SORT_CHUNK;
 	const u64 t0 = wall_clock64();
 	const ReadOut o = align_read<1, false>(g, rd.seq + rd.off[r], rd.ql[r], (u32)rd.v0[r], rd.v1, w, w, b, vis, nullptr, r);
 	write_out<1>(o, r, t0, out_ed, out_status, out_counts, b);
@@ -952,6 +1041,8 @@ __global__ __launch_bounds__(WAVE *NW) void gwf_persistent_kernel(DGraph g, DRea
 	const i32 q_lo = bin_off[bin], q_n = bin_off[bin + 1] - q_lo;
 	hb.sort_lds = arena;  // the LDS arena is free once a read has spilled
 	hb.sort_cap = arena_sort_cap(lc);
+	hb.sort_n = (i32)lds_elems(lc);
+	hb.mw = mw;
 	GwfBeat *beat = beats ? beats + blockIdx.x : nullptr;
 	for (;;) {
 		if (b.tid() == 0) next_q = atomicAdd(&heads[bin], 1);
@@ -1068,6 +1159,8 @@ __global__ __launch_bounds__(WAVE *NW) void gwf_serve_kernel(SrvArgs a) {
 		Bufs hb = carve((u64 *)(a.pool + e.d_off + e.ws_off), e.hc, true);
 		hb.sort_lds = arena;
 		hb.sort_cap = sort_cap;
+		hb.sort_n = (i32)lds_elems(a.lc);
+		hb.mw = a.mw;
 		const u64 t0 = wall_clock64();
 		const ReadOut o = align_read<NW, true>(a.g, dq, e.ql, (u32)e.v0, -1, lb, hb, b, vis, beat, (i32)seq);
 		int64_t c = o.chars;
