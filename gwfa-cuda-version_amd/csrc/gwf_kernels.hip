@@ -413,6 +413,41 @@ __device__ __forceinline__ void merge_intervals2(const u64 *alo, const u64 *ahi,
 	b.sync();
 }
 
+// The same merge, partitioned into contiguous ranges of the output: one
+// merge-path search per thread, then a sequential merge (for lists in HBM,
+// where a binary search per element is a chain of dependent loads).
+template <int NW>
+__device__ __forceinline__ void merge_intervals_ranges(const u64 *alo, const u64 *ahi, i32 na, const u64 *clo,
+                                                       const u64 *chi, i32 nc, u64 *olo, u64 *ohi, const Blk<NW> &b) {
+	constexpr i32 NT = Blk<NW>::NT;
+	const i32 n = na + nc;
+	const i32 E = (n + NT - 1) / NT;
+	const i32 o0 = (i32)b.tid() * E;
+	const i32 o1 = o0 + E < n ? o0 + E : n;
+	if (o0 < n) {
+		i32 lo = o0 - nc > 0 ? o0 - nc : 0, up = o0 < na ? o0 : na;
+		while (lo < up) {
+			const i32 mid = (lo + up) >> 1;
+			if (alo[mid] <= clo[o0 - 1 - mid]) lo = mid + 1;
+			else up = mid;
+		}
+		i32 i = lo, j = o0 - lo;
+		u64 ah = i < na ? alo[i] : EMPTY_KEY, ch = j < nc ? clo[j] : EMPTY_KEY;
+		for (i32 o = o0; o < o1; ++o) {
+			if (i < na && (j >= nc || ah <= ch)) {
+				olo[o] = ah, ohi[o] = ahi[i];
+				++i;
+				ah = i < na ? alo[i] : EMPTY_KEY;
+			} else {
+				olo[o] = ch, ohi[o] = chi[j];
+				++j;
+				ch = j < nc ? clo[j] : EMPTY_KEY;
+			}
+		}
+	}
+	b.sync();
+}
+
 // Coalesce overlapping or adjacent intervals of a list sorted by lo
 // (gwf_intv_merge_adj, gwf-ed.cu:266-296).  Returns the output count (the
 // caller guarantees room for n).
@@ -750,7 +785,8 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 		}
 		b.sync();
 		u64 *Clo = w.P + 2 * n_tt, *Chi = Clo + (n_h + n_ti);
-		merge_intervals2(Hcur_lo, Hcur_hi, n_h, TIlo, TIhi, n_ti, Clo, Chi, b);
+		if constexpr (HBM) merge_intervals_ranges(Hcur_lo, Hcur_hi, n_h, TIlo, TIhi, n_ti, Clo, Chi, b);
+		else merge_intervals2(Hcur_lo, Hcur_hi, n_h, TIlo, TIhi, n_ti, Clo, Chi, b);
 		nh2 = coalesce(Clo, Chi, n_h + n_ti, Hlo, Hhi, b);
 	}
 
