@@ -144,9 +144,11 @@ def main():
     ap.add_argument("--config", type=int, default=2, help="workload 1-4 (default 2)")
     ap.add_argument("--kernel", choices=["plain", "persistent"], default="persistent")
     ap.add_argument("--reads", type=int, default=None, help="use only the first N reads")
-    ap.add_argument("--cpu-sample", type=int, default=3000)
+    ap.add_argument("--cpu-sample", type=int, default=20000,
+                    help="reads timed on the CPU oracle and parity-checked (default: the whole config-2 workload)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tune", default=None, help="persistent kernel: load this bin table (no autotuning)")
+    ap.add_argument("--save-tune", default=None, help="persistent kernel: write the table used to this file")
     ap.add_argument("--autotune", type=int, default=-1,
                     help="persistent kernel: autotune the bin table on the first N reads before timing, untimed "
                          "(-1: all reads; 0: use the built-in table)")
@@ -191,6 +193,9 @@ def main():
         log(f"autotune: {time.perf_counter() - t:.1f} s (untimed)")
     elif kernel == gw.KERNEL_PERSISTENT:
         tuning = gw.tune_get()
+
+    if args.save_tune and kernel == gw.KERNEL_PERSISTENT:
+        gw.tune_save(args.save_tune)
 
     for i in range(args.warmup):
         eds0, st = gw.ed_run(dg, dr, kernel=kernel)

@@ -545,6 +545,39 @@ __device__ inline Bufs carve(u64 *p, const Caps &c, bool with_tmp) {
 	return w;
 }
 
+// In-vertex successors of the state cur = (v, d, k) (the extend_batch
+// recurrence, gwf-ed.cu:913-963): deletion -> (d-1, k+1), mismatch ->
+// (d, k+1), insertion -> (d+1, k).  The state owns output diagonal d-1
+// unless its left neighbour (same vertex) is on d-1 or d-2, d+1 unless its
+// right neighbour is on d+1, and d always; an owner takes the max over the
+// candidates of its neighbours too.  s* = kept in the vertex and the query,
+// t* = leaves the vertex (an exit for the history).
+struct Succ {
+	bool s0, s1, s2, t0, t1, t2;
+	i32 k0, k1, k2;
+};
+__device__ __forceinline__ Succ successors(bool act, u64 cur, u64 prv, u64 nxt, i32 vl, i32 ql) {
+	const u32 v = key_v(cur);
+	const i32 d = key_d(cur), k = key_k(cur);
+	const bool hp = act && key_v(prv) == v;  // EMPTY_KEY never matches a vertex
+	const bool hn = act && key_v(nxt) == v;
+	const i32 dp = key_d(prv), kp = key_k(prv), dn = key_d(nxt), kn = key_k(nxt);
+	Succ o;
+	const bool e0 = act && !(hp && dp >= d - 2);
+	o.k0 = k + 1;
+	o.k1 = k + 1;
+	if (hp && dp == d - 1 && kp > o.k1) o.k1 = kp;
+	if (hn && dn == d + 1 && kn + 1 > o.k1) o.k1 = kn + 1;
+	const bool e1 = act;
+	const bool e2 = act && !(hn && dn == d + 1);
+	o.k2 = k;
+	if (hn && dn == d + 2 && kn + 1 > o.k2) o.k2 = kn + 1;
+	o.s0 = e0 && (d - 1) + o.k0 < ql && o.k0 < vl, o.t0 = e0 && o.k0 == vl;
+	o.s1 = e1 && d + o.k1 < ql && o.k1 < vl, o.t1 = e1 && o.k1 == vl;
+	o.s2 = e2 && (d + 1) + o.k2 < ql && o.k2 < vl, o.t2 = e2 && o.k2 == vl;
+	return o;
+}
+
 enum { STEP_NEXT = 0, STEP_FOUND = 1, STEP_EMPTY = 2, STEP_SPILL = 3 };
 
 // One score step.  Reads P[0..n_a) (the wavefront, sorted) and the history
@@ -559,76 +592,131 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 	const i32 tid = (i32)b.tid();
 	const Caps &cap = w.c;
 
-	// ---- P1: extend, detect the query end, collect vertex ends (batch frontier)
+	// ---- P1: extend, detect the end, collect vertex ends (the closure's frontier).
+	// Each thread takes S1 states per round (strided, so loads coalesce) and
+	// issues the loads of all of them before using any: one chain of
+	// dependent loads (key, vertex record, first 8 bytes) per round instead
+	// of one per state.  States still matching after 8 bytes continue in
+	// extend_wave (16-byte lane steps, then 512-byte cooperative wave steps).
+	constexpr i32 S1 = 2;
 	bool found = false;
 	i32 n_f = 0;
-	for (i32 base = 0; base < n_a; base += NT) {
-		const i32 j = base + tid;
-		const bool act = j < n_a;
-		const u64 key = act ? w.P[j] : 0;
-		const u32 v = key_v(key);
-		const i32 d = key_d(key);
-		i32 k = key_k(key);
-		const uint4 vi = act ? g.vinfo[v] : make_uint4(0, 0, 0, 0);
-		const i32 vl = (i32)vi.y;
-		const i32 max_k = (ql - d < vl ? ql - d : vl) - 1;
-		k = extend_wave(act, g.seq, vi.x, q, d, k, max_k, &chars);
-		if (act) w.P[j] = mk(v, d, k);
-		const bool qend = act && d + k == ql - 1;
-		// the end (gwf-ed.cu:1231-1236): anywhere, or at the last base of v1
-		const bool fnd = qend && (v1 < 0 || (v == (u32)v1 && k == vl - 1));
-		found |= fnd;
-		// vertex ends go to the closure; with v1 >= 0 that includes query
-		// ends, which continue by deletion into the next vertices
-		const bool vend = act && !fnd && k == vl - 1;
-		const i32 sl = b.slot(vend, &n_f);
-		if (vend && sl < cap.f) w.F0[sl] = mk(v, d, k);
+	for (i32 base = 0; base < n_a; base += NT * S1) {
+		u64 key[S1];
+		uint2 vi[S1];
+#pragma unroll
+		for (i32 s = 0; s < S1; ++s) {
+			const i32 j = base + s * NT + tid;
+			key[s] = j < n_a ? w.P[j] : EMPTY_KEY;
+		}
+#pragma unroll
+		for (i32 s = 0; s < S1; ++s)
+			vi[s] = key[s] != EMPTY_KEY ? *(const uint2 *)&g.vinfo[key_v(key[s])] : make_uint2(0, 0);
+		i32 kk[S1], mk8[S1];
+		u64 x8[S1];
+#pragma unroll
+		for (i32 s = 0; s < S1; ++s) {
+			const i32 d = key_d(key[s]), vl = (i32)vi[s].y;
+			kk[s] = key_k(key[s]);
+			mk8[s] = (ql - d < vl ? ql - d : vl) - 1;
+			const bool go = key[s] != EMPTY_KEY && mk8[s] > kk[s];
+			x8[s] = go ? load8u(g.seq + vi[s].x + kk[s] + 1) ^ load8u(q + d + kk[s] + 1) : 0;
+		}
+		bool more[S1];
+#pragma unroll
+		for (i32 s = 0; s < S1; ++s) {
+			more[s] = false;
+			const i32 avail = mk8[s] - kk[s];
+			if (key[s] != EMPTY_KEY && avail > 0) {
+				const i32 m = x8[s] ? (i32)(__builtin_ctzll(x8[s]) >> 3) : 8;
+				if (m >= avail) { chars += avail; kk[s] = mk8[s]; }
+				else if (m < 8) { chars += m + 1; kk[s] += m; }
+				else { chars += 8; kk[s] += 8; more[s] = true; }
+			}
+		}
+#pragma unroll
+		for (i32 s = 0; s < S1; ++s)  // every thread calls it S1 times (it is a wave collective)
+			kk[s] = extend_wave(more[s], g.seq, vi[s].x, q, key_d(key[s]), kk[s], mk8[s], &chars);
+		bool vend[S1];
+		i32 nv = 0;
+#pragma unroll
+		for (i32 s = 0; s < S1; ++s) {
+			const bool act = key[s] != EMPTY_KEY;
+			const u32 v = key_v(key[s]);
+			const i32 d = key_d(key[s]), k = kk[s], vl = (i32)vi[s].y;
+			if (act) w.P[base + s * NT + tid] = mk(v, d, k);
+			const bool qend = act && d + k == ql - 1;
+			// the end (gwf-ed.cu:1231-1236): anywhere, or at the last base of v1
+			const bool fnd = qend && (v1 < 0 || (v == (u32)v1 && k == vl - 1));
+			found |= fnd;
+			// vertex ends go to the closure; with v1 >= 0 that includes query
+			// ends, which continue by deletion into the next vertices
+			vend[s] = act && !fnd && k == vl - 1;
+			nv += vend[s];
+		}
+		i32 tot;
+		i32 pos = n_f + b.scan(nv, &tot);
+#pragma unroll
+		for (i32 s = 0; s < S1; ++s)
+			if (vend[s]) {
+				if (pos < cap.f) w.F0[pos] = mk(key_v(key[s]), key_d(key[s]), kk[s]);
+				++pos;
+			}
+		n_f += tot;
 	}
 	if (b.any(found)) return STEP_FOUND;
 	if (n_f > cap.f) return STEP_SPILL;
 
-	// ---- P2: in-vertex successors into Q (sorted) and their exits into T
+	// ---- P2: in-vertex successors into Q (sorted) and their exits into T.
+	// Each thread takes S2 consecutive states per round (plus one neighbour on
+	// each side), so the outputs, written in thread order, stay sorted.
+	constexpr i32 S2 = 2;
 	i32 n_s = 0, n_t = 0;
 	bool ovf = false;
-	for (i32 base = 0; base < n_a; base += NT) {
-		const i32 j = base + tid;
-		const bool act = j < n_a;
-		const u64 cur = act ? w.P[j] : 0;
-		const u64 prv = (act && j > 0) ? w.P[j - 1] : EMPTY_KEY;
-		const u64 nxt = (act && j + 1 < n_a) ? w.P[j + 1] : EMPTY_KEY;
-		const u32 v = key_v(cur);
-		const i32 d = key_d(cur), k = key_k(cur);
-		const i32 vl = act ? (i32)g.vinfo[v].y : 0;
-		const bool hp = act && j > 0 && key_v(prv) == v;
-		const bool hn = act && j + 1 < n_a && key_v(nxt) == v;
-		const i32 dp = key_d(prv), kp = key_k(prv), dn = key_d(nxt), kn = key_k(nxt);
-		const bool e0 = act && !(hp && dp >= d - 2);
-		const i32 k0 = k + 1;
-		i32 k1 = k + 1;
-		if (hp && dp == d - 1 && kp > k1) k1 = kp;
-		if (hn && dn == d + 1 && kn + 1 > k1) k1 = kn + 1;
-		const bool e1 = act;
-		const bool e2 = act && !(hn && dn == d + 1);
-		i32 k2 = k;
-		if (hn && dn == d + 2 && kn + 1 > k2) k2 = kn + 1;
-		const bool s0 = e0 && (d - 1) + k0 < ql && k0 < vl, t0 = e0 && k0 == vl;
-		const bool s1 = e1 && d + k1 < ql && k1 < vl, t1 = e1 && k1 == vl;
-		const bool s2 = e2 && (d + 1) + k2 < ql && k2 < vl, t2 = e2 && k2 == vl;
+	for (i32 base = 0; base < n_a; base += NT * S2) {
+		const i32 j0 = base + tid * S2;
+		u64 c[S2 + 2];  // c[0] = P[j0 - 1] ... c[S2 + 1] = P[j0 + S2]
+#pragma unroll
+		for (i32 s = 0; s < S2 + 2; ++s) {
+			const i32 j = j0 - 1 + s;
+			c[s] = (j >= 0 && j < n_a && j < base + NT * S2 + 1) ? w.P[j] : EMPTY_KEY;
+		}
+		i32 vl[S2];
+#pragma unroll
+		for (i32 s = 0; s < S2; ++s) vl[s] = (j0 + s < n_a) ? (i32)g.vinfo[key_v(c[s + 1])].y : 0;
+		i32 cs = 0, ct = 0;
+		Succ sc[S2];
+#pragma unroll
+		for (i32 s = 0; s < S2; ++s) {
+			sc[s] = successors(j0 + s < n_a, c[s + 1], c[s], c[s + 2], vl[s], ql);
+			cs += (i32)sc[s].s0 + (i32)sc[s].s1 + (i32)sc[s].s2;
+			ct += (i32)sc[s].t0 + (i32)sc[s].t1 + (i32)sc[s].t2;
+		}
 		i32 tot;
-		i32 p = n_s + b.scan((i32)s0 + (i32)s1 + (i32)s2, &tot);
+		i32 p = n_s + b.scan(cs, &tot);
 		if (n_s + tot > cap.m) ovf = true;
 		else {
-			if (s0) w.Q[p++] = mk(v, d - 1, k0);
-			if (s1) w.Q[p++] = mk(v, d, k1);
-			if (s2) w.Q[p++] = mk(v, d + 1, k2);
+#pragma unroll
+			for (i32 s = 0; s < S2; ++s) {
+				const u32 v = key_v(c[s + 1]);
+				const i32 d = key_d(c[s + 1]);
+				if (sc[s].s0) w.Q[p++] = mk(v, d - 1, sc[s].k0);
+				if (sc[s].s1) w.Q[p++] = mk(v, d, sc[s].k1);
+				if (sc[s].s2) w.Q[p++] = mk(v, d + 1, sc[s].k2);
+			}
 		}
 		n_s += tot;
-		p = n_t + b.scan((i32)t0 + (i32)t1 + (i32)t2, &tot);
+		p = n_t + b.scan(ct, &tot);
 		if (n_t + tot > cap.t) ovf = true;
 		else {
-			if (t0) w.T[p++] = diag_id(v, d - 1);
-			if (t1) w.T[p++] = diag_id(v, d);
-			if (t2) w.T[p++] = diag_id(v, d + 1);
+#pragma unroll
+			for (i32 s = 0; s < S2; ++s) {
+				const u32 v = key_v(c[s + 1]);
+				const i32 d = key_d(c[s + 1]);
+				if (sc[s].t0) w.T[p++] = diag_id(v, d - 1);
+				if (sc[s].t1) w.T[p++] = diag_id(v, d);
+				if (sc[s].t2) w.T[p++] = diag_id(v, d + 1);
+			}
 		}
 		n_t += tot;
 	}
@@ -832,6 +920,7 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 		const i32 E = (n_m + NT - 1) / NT;
 		const i32 o0 = tid * E;
 		const i32 o1 = o0 + E < n_m ? o0 + E : n_m;
+		i32 kept = 0;
 		if (o0 < n_m) {
 			i32 lo = o0 - n_x > 0 ? o0 - n_x : 0, up = o0 < n_s ? o0 : n_s;
 			while (lo < up) {
@@ -874,17 +963,12 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 					}
 					keep = !(h < nh2 && Hl[h] <= dg);  // coalesced intervals: disjoint, sorted
 				}
-				M[o] = keep ? x : EMPTY_KEY;
+				if (keep) M[o0 + kept++] = x;  // kept states packed at the front of the range
 			}
 		}
-		b.sync();
-		for (i32 base = 0; base < n_m; base += NT) {
-			const i32 j = base + tid;
-			const u64 x = j < n_m ? M[j] : EMPTY_KEY;
-			const bool keep = x != EMPTY_KEY;
-			const i32 sl = b.slot(keep, &n_new);
-			if (keep) w.P[sl] = x;
-		}
+		// one scan of the per-thread counts; each thread copies its own kept states
+		const i32 at = b.scan(kept, &n_new);
+		for (i32 e = 0; e < kept; ++e) w.P[at + e] = M[o0 + e];
 		b.sync();
 	} else {
 		// ---- P6: merge the sorted X into Q in place (merge path, right to left; Q first on ties)
@@ -1059,8 +1143,12 @@ __device__ __forceinline__ i32 arena_sort_cap(const Caps &lc) {
 // The queue of bin `bin` is order[bin_off[bin] .. bin_off[bin + 1]) (device
 // memory, so a device-side ordering pass can fill it); heads[bin] counts the
 // reads taken from it.  order == nullptr: the identity order.
+// The step is latency-bound: resident waves matter more than a few spilled
+// registers.  Capping registers at 5 waves per SIMD (96 VGPRs, a few dozen
+// bytes per lane spilled) measured 9-15% faster than the uncapped 4 waves.
 template <int NW>
-__global__ __launch_bounds__(WAVE *NW) void gwf_persistent_kernel(DGraph g, DReads rd, const i32 *order,
+__global__ __launch_bounds__(WAVE *NW) __attribute__((amdgpu_waves_per_eu(5))) void gwf_persistent_kernel(
+    DGraph g, DReads rd, const i32 *order,
                                                                  const i32 *bin_off, i32 bin, i32 *heads, Caps lc,
                                                                  Caps hc, i32 mw, u64 *scratch, u64 ws_elems_per_wg,
                                                                  i32 *out_ed, i32 *out_status, int64_t *out_counts,
@@ -1112,7 +1200,7 @@ __device__ __forceinline__ u64 host_ld(const u64 *p) {
 }
 
 template <int NW>
-__global__ __launch_bounds__(WAVE *NW) void gwf_serve_kernel(SrvArgs a) {
+__global__ __launch_bounds__(WAVE *NW) __attribute__((amdgpu_waves_per_eu(5))) void gwf_serve_kernel(SrvArgs a) {
 	extern __shared__ u64 arena[];
 	__shared__ u64 vis[VIS_SLOTS];
 	__shared__ i32 red[NW];

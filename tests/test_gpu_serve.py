@@ -68,6 +68,7 @@ def test_serve_alone_and_in_mix(gpu, c3):
         assert st["completed"] == st["submitted"] == 12 + 2 * len(r)
         assert st["inflight_peak"] > 1          # requests overlapped on the device
         assert st["budget_peak"] <= st["budget_bytes"]
+        assert st["stalled"] == 0
     finally:
         dg.free()
 

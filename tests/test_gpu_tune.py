@@ -87,3 +87,25 @@ def test_autotune(gpu, mix):
         gpu.tune_set(None)
         dr.free()
         dg.free()
+
+
+def test_heartbeat_progress_lines(gpu, capfd, monkeypatch):
+    """GWF_HEARTBEAT_MS: a batch run prints progress lines from the
+    workgroups' records while it runs (main.cu:48-149's progress polling)."""
+    _say("case: heartbeat lines during a binned run")
+    g, r = gpu.make_workload(4, 300)
+    g.index()
+    dg, dr = gpu.DeviceGraph(g), gpu.DeviceReads(r)
+    try:
+        monkeypatch.setenv("GWF_HEARTBEAT_MS", "2")
+        got, _ = gpu.ed_run(dg, dr, kernel=gpu.KERNEL_PERSISTENT)
+        monkeypatch.delenv("GWF_HEARTBEAT_MS")
+        want, _ = gpu.ed_run(dg, dr, kernel=gpu.KERNEL_PERSISTENT)
+    finally:
+        dr.free()
+        dg.free()
+    err = capfd.readouterr().err
+    lines = [l for l in err.splitlines() if l.startswith("[gwf heartbeat]")]
+    assert len(lines) >= 2, err[-2000:]
+    assert "busy=" in lines[0] and "max_step=" in lines[0]
+    assert np.array_equal(got, want)
