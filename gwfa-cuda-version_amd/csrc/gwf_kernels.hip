@@ -1200,7 +1200,7 @@ __device__ __forceinline__ u64 host_ld(const u64 *p) {
 }
 
 template <int NW>
-__global__ __launch_bounds__(WAVE *NW) __attribute__((amdgpu_waves_per_eu(5))) void gwf_serve_kernel(SrvArgs a) {
+__global__ __launch_bounds__(WAVE *NW) void gwf_serve_kernel(SrvArgs a) {
 	extern __shared__ u64 arena[];
 	__shared__ u64 vis[VIS_SLOTS];
 	__shared__ i32 red[NW];

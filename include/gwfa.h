@@ -252,8 +252,8 @@ typedef struct {
 	int64_t state_budget_bytes; /* device bytes for admitted requests (0: 1 GiB) */
 	int64_t staging_bytes;      /* pinned host bytes for admitted reads (0: 64 MiB) */
 	int32_t ring_slots;         /* most requests on the device at once, power of two (0: 4096) */
-	int32_t nw;                 /* waves per read: 1, 2 or 4 (0: 2) */
-	int32_t lds_states;         /* LDS wavefront capacity per read (0: 256) */
+	int32_t nw;                 /* waves per read: 1, 2 or 4 (0: 4) */
+	int32_t lds_states;         /* LDS wavefront capacity per read (0: 1024) */
 	int32_t wgs;                /* resident workgroups (0: occupancy x CUs) */
 	int32_t park_ms;            /* idle time after which the device loop is parked (0: 20) */
 	int32_t watchdog_ms;        /* heartbeat silence that counts a busy workgroup as stalled (0: 10000) */
