@@ -67,7 +67,7 @@ struct PersistCfg {
 	i32 wgs;         // resident workgroups (grid size)
 	Caps lds;        // LDS-resident capacities
 	Caps hbm;        // HBM spill capacities
-	size_t lds_bytes;
+	size_t lds_bytes;  // dynamic LDS: the LDS-resident arrays, grown to the occupancy's share (HBM-mode staging)
 	i32 mw;          // LDS sort: at most mw runs of 64 are merged by one rank pass
 };
 
@@ -113,6 +113,7 @@ size_t gwf_max_lds_bytes();
 
 // Resident persistent workgroups per CU for nw waves and lds_bytes of LDS.
 int gwf_persist_occupancy(int nw, size_t lds_bytes);
+size_t gwf_persist_grow_lds(int nw, size_t lds_bytes, int per_cu, size_t margin);
 
 // ------------------------------------------------------------ heartbeat
 // Progress record of one workgroup, written by the kernel into host-coherent

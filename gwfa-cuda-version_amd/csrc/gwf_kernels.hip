@@ -448,6 +448,26 @@ __device__ __forceinline__ void merge_intervals_ranges(const u64 *alo, const u64
 	b.sync();
 }
 
+// Runs of consecutive ids in the sorted, duplicate-free T[0..n) as half-open
+// intervals [lo, hi); returns their count.
+template <int NW>
+__device__ __forceinline__ i32 ti_runs(const u64 *T, i32 n, u64 *lo, u64 *hi, const Blk<NW> &b) {
+	i32 n_ti = 0;
+	for (i32 base = 0; base < n; base += Blk<NW>::NT) {
+		const i32 j = base + (i32)b.tid();
+		const bool act = j < n;
+		const u64 x = act ? T[j] : 0;
+		const bool start = act && (j == 0 || T[j - 1] + 1 != x);
+		const bool end = act && (j + 1 == n || T[j + 1] != x + 1);
+		i32 cnt;
+		const i32 e = b.scan(start ? 1 : 0, &cnt);
+		if (start) lo[n_ti + e] = x;
+		if (end) hi[n_ti + e + (start ? 1 : 0) - 1] = x + 1;
+		n_ti += cnt;
+	}
+	return n_ti;
+}
+
 // Coalesce overlapping or adjacent intervals of a list sorted by lo
 // (gwf_intv_merge_adj, gwf-ed.cu:266-296).  Returns the output count (the
 // caller guarantees room for n).
@@ -481,6 +501,100 @@ __device__ __forceinline__ i32 coalesce(const u64 *lo, const u64 *hi, i32 n, u64
 __device__ __forceinline__ bool forbidden(const u64 *hlo, const u64 *hhi, i32 nh, u64 dg) {
 	const i32 p = upper_bound(hlo, nh, dg) - 1;
 	return p >= 0 && dg < hhi[p];
+}
+
+// LDS-qualified view of a pointer into the arena: loads through it compile
+// to LDS instructions instead of flat ones.
+typedef const __attribute__((address_space(3))) u64 *lds_cptr;
+
+// dst[0..n) = src[0..n), four loads in flight per thread (no barrier).
+template <int NW>
+__device__ __forceinline__ void stage(u64 *dst, const u64 *src, i32 n, const Blk<NW> &b) {
+	constexpr i32 NT = Blk<NW>::NT;
+	for (i32 j0 = (i32)b.tid(); j0 < n; j0 += 4 * NT) {
+		u64 v[4];
+#pragma unroll
+		for (i32 r = 0; r < 4; ++r) {
+			const i32 j = j0 + r * NT;
+			v[r] = j < n ? src[j] : 0;
+		}
+#pragma unroll
+		for (i32 r = 0; r < 4; ++r) {
+			const i32 j = j0 + r * NT;
+			if (j < n) dst[j] = v[r];
+		}
+	}
+}
+
+// Merge of the sorted successor lists Q (n_s) and X (n_x) fused with the
+// furthest-state-per-diagonal rule and the history test (P6+P7 for arrays
+// in HBM).  Thread t takes the contiguous range [t E, t E + E) of the merged
+// order: one merge-path search gives its start in Q and X; it then merges
+// sequentially (Q first on ties), keeps the first (furthest) state of each
+// diagonal and tests the history (coalesced intervals Hl/Hh, nh of them)
+// with a pointer that only moves forward.  Kept states are packed at the
+// front of each range in M, then compacted into P; returns their count.
+// QP and XP are lds_cptr or const u64 * (where each list lives).
+template <int NW, typename QP, typename XP>
+__device__ __forceinline__ i32 merge_filter(QP Q, i32 n_s, XP X, i32 n_x, const u64 *Hl, const u64 *Hh, i32 nh, u64 *M,
+                                            u64 *P, const Blk<NW> &b) {
+	constexpr i32 NT = Blk<NW>::NT;
+	const i32 n_m = n_s + n_x;
+	const i32 E = (n_m + NT - 1) / NT;
+	const i32 o0 = (i32)b.tid() * E;
+	const i32 o1 = o0 + E < n_m ? o0 + E : n_m;
+	i32 kept = 0;
+	if (o0 < n_m) {
+		i32 lo = o0 - n_x > 0 ? o0 - n_x : 0, up = o0 < n_s ? o0 : n_s;
+		while (lo < up) {
+			const i32 mid = (lo + up) >> 1;
+			if (Q[mid] <= X[o0 - 1 - mid]) lo = mid + 1;
+			else up = mid;
+		}
+		i32 i = lo, jx = o0 - lo;
+		u64 prev_dg = ~0ull;  // diag of the merged state before o0 (none: o0 == 0)
+		if (o0 > 0) {
+			const bool from_q = i > 0 && (jx == 0 || Q[i - 1] > X[jx - 1]);
+			prev_dg = key_diag(from_q ? Q[i - 1] : X[jx - 1]);
+		}
+		u64 qh = i < n_s ? Q[i] : EMPTY_KEY, xh = jx < n_x ? X[jx] : EMPTY_KEY;
+		i32 h = -1;
+		for (i32 o = o0; o < o1; ++o) {
+			const bool takeq = i < n_s && (jx >= n_x || qh <= xh);
+			const u64 x = takeq ? qh : xh;
+			if (takeq) {
+				++i;
+				qh = i < n_s ? Q[i] : EMPTY_KEY;
+			} else {
+				++jx;
+				xh = jx < n_x ? X[jx] : EMPTY_KEY;
+			}
+			const u64 dg = key_diag(x);
+			bool keep = dg != prev_dg;
+			prev_dg = dg;
+			if (keep) {
+				if (h < 0) {  // first interval ending after dg
+					i32 a = 0, c = nh;
+					while (a < c) {
+						const i32 mid = (a + c) >> 1;
+						if (Hh[mid] <= dg) a = mid + 1;
+						else c = mid;
+					}
+					h = a;
+				} else {
+					while (h < nh && Hh[h] <= dg) ++h;
+				}
+				keep = !(h < nh && Hl[h] <= dg);  // coalesced intervals: disjoint, sorted
+			}
+			if (keep) M[o0 + kept++] = x;
+		}
+	}
+	// one scan of the per-thread counts; each thread copies its own kept states
+	i32 n_new;
+	const i32 at = b.scan(kept, &n_new);
+	for (i32 e = 0; e < kept; ++e) P[at + e] = M[o0 + e];
+	b.sync();
+	return n_new;
 }
 
 // ------------------------------------------------------------ visited set
@@ -856,21 +970,28 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 	u64 *Hcur_lo = hs ? w.H1lo : w.H0lo, *Hcur_hi = hs ? w.H1hi : w.H0hi;
 	u64 *Hlo = hs ? w.H0lo : w.H1lo, *Hhi = hs ? w.H0hi : w.H1hi;
 	i32 nh2;
-	{
+	i32 hist_res = 0;                    // HBM mode: LDS elements reserved at the top of the scratch
+	const u64 *Hl_f = Hlo, *Hh_f = Hhi;  // where the filter reads the new history
+	const i32 cap_h = n_h + n_tt;        // bound on the merged interval count
+	if (HBM && 2 * n_tt + 2 * n_h + 4 * cap_h <= w.sort_n) {
+		// in the LDS scratch: exit runs, current history, merged list; the new
+		// history at the top, where it stays for the filter (and is copied to
+		// HBM for the next step)
+		u64 *l = w.sort_lds;
+		u64 *hc = l + 2 * n_tt, *Clo = hc + 2 * n_h, *Chi = Clo + cap_h;
+		u64 *Olo = l + w.sort_n - 2 * cap_h, *Ohi = Olo + cap_h;
+		stage(hc, Hcur_lo, n_h, b);
+		stage(hc + n_h, Hcur_hi, n_h, b);
+		const i32 n_ti = ti_runs(w.T, n_tt, l, l + n_tt, b);
+		b.sync();
+		merge_intervals2(hc, hc + n_h, n_h, l, l + n_tt, n_ti, Clo, Chi, b);
+		nh2 = coalesce(Clo, Chi, n_h + n_ti, Olo, Ohi, b);
+		for (i32 t = tid; t < nh2; t += NT) Hlo[t] = Olo[t], Hhi[t] = Ohi[t];
+		hist_res = 2 * cap_h;
+		Hl_f = Olo, Hh_f = Ohi;
+	} else {
 		u64 *TIlo = w.P, *TIhi = w.P + n_tt;
-		i32 n_ti = 0;
-		for (i32 base = 0; base < n_tt; base += NT) {
-			const i32 j = base + tid;
-			const bool act = j < n_tt;
-			const u64 x = act ? w.T[j] : 0;
-			const bool start = act && (j == 0 || w.T[j - 1] + 1 != x);
-			const bool end = act && (j + 1 == n_tt || w.T[j + 1] != x + 1);
-			i32 cnt;
-			const i32 e = b.scan(start ? 1 : 0, &cnt);
-			if (start) TIlo[n_ti + e] = x;
-			if (end) TIhi[n_ti + e + (start ? 1 : 0) - 1] = x + 1;
-			n_ti += cnt;
-		}
+		const i32 n_ti = ti_runs(w.T, n_tt, TIlo, TIhi, b);
 		b.sync();
 		u64 *Clo = w.P + 2 * n_tt, *Chi = Clo + (n_h + n_ti);
 		if constexpr (HBM) merge_intervals_ranges(Hcur_lo, Hcur_hi, n_h, TIlo, TIhi, n_ti, Clo, Chi, b);
@@ -880,15 +1001,22 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 
 	// ---- P4: sort the closure successors into P
 	const u64 *Xs;
-	i32 lds_used = 0;  // HBM mode: elements of the LDS scratch holding the sorted X
+	i32 lds_used = 0;     // HBM mode: elements of the LDS scratch holding the sorted X
+	bool x_lds = false;  // HBM mode: the sorted X is in the LDS scratch
 	if constexpr (HBM) {
-		if (2 * n_x <= w.sort_n) {  // stage X in LDS, sort it there and merge from there
-			for (i32 t = tid; t < n_x; t += NT) w.sort_lds[t] = w.X[t];
+		if (2 * n_x <= w.sort_n - hist_res) {  // stage X in LDS, sort it there and merge from there
+			stage(w.sort_lds, w.X, n_x, b);
 			b.sync();
 			sort_runs(w.sort_lds, n_x, w.sort_lds + n_x, w.mw, b);
 			Xs = w.sort_lds + n_x;
 			lds_used = 2 * n_x;
+			x_lds = true;
 		} else {
+			if (hist_res) {  // the sort takes the whole scratch: the filter reads the history from HBM
+				b.sync();
+				hist_res = 0;
+				Hl_f = Hlo, Hh_f = Hhi;
+			}
 			sort_hbm(w.X, n_x, w.sort_lds, w.sort_cap, w.sort_tmp, b);
 			Xs = w.X;
 		}
@@ -900,76 +1028,25 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 	const i32 n_m = n_s + n_x;
 	i32 n_new = 0;
 	if (HBM && n_m <= 2 * cap.f) {
-		// ---- P6+P7 fused (arrays in HBM): thread t takes the contiguous range
-		// [t E, t E + E) of the merged order.  One merge-path search gives its
-		// start in Q and X; it then merges sequentially (Q first on ties),
-		// keeps the first (furthest) state of each diagonal, and tests the
-		// history with a pointer that only moves forward (one search for the
-		// first state).  Outputs go to F0..F1 (EMPTY_KEY when dropped) and are
-		// compacted into P.  A handful of dependent loads per thread instead of
-		// a binary search in HBM per state.
+		// ---- P6+P7 fused (arrays in HBM), from LDS copies where they fit:
+		// the sorted X is there already; the new history and Q follow it.
 		u64 *M = w.F0;  // F0 and F1 are adjacent: 2 cap.f >= n_m
-		// the new history in LDS too, when it fits beside the sorted X
-		const u64 *Hl = Hlo, *Hh = Hhi;
-		if (lds_used + 2 * nh2 <= w.sort_n) {
-			u64 *hl = w.sort_lds + lds_used, *hh = hl + nh2;
-			for (i32 t = tid; t < nh2; t += NT) hl[t] = Hlo[t], hh[t] = Hhi[t];
-			b.sync();
+		const u64 *Hl = Hl_f, *Hh = Hh_f;
+		i32 used = lds_used;
+		const i32 avail = w.sort_n - hist_res;
+		if (!hist_res && used + 2 * nh2 <= avail) {
+			u64 *hl = w.sort_lds + used, *hh = hl + nh2;
+			stage(hl, Hlo, nh2, b);
+			stage(hh, Hhi, nh2, b);
 			Hl = hl, Hh = hh;
+			used += 2 * nh2;
 		}
-		const i32 E = (n_m + NT - 1) / NT;
-		const i32 o0 = tid * E;
-		const i32 o1 = o0 + E < n_m ? o0 + E : n_m;
-		i32 kept = 0;
-		if (o0 < n_m) {
-			i32 lo = o0 - n_x > 0 ? o0 - n_x : 0, up = o0 < n_s ? o0 : n_s;
-			while (lo < up) {
-				const i32 mid = (lo + up) >> 1;
-				if (w.Q[mid] <= Xs[o0 - 1 - mid]) lo = mid + 1;
-				else up = mid;
-			}
-			i32 i = lo, jx = o0 - lo;
-			u64 prev_dg = ~0ull;  // diag of the merged state before o0 (none: o0 == 0)
-			if (o0 > 0) {
-				const bool from_q = i > 0 && (jx == 0 || w.Q[i - 1] > Xs[jx - 1]);
-				prev_dg = key_diag(from_q ? w.Q[i - 1] : Xs[jx - 1]);
-			}
-			u64 qh = i < n_s ? w.Q[i] : EMPTY_KEY, xh = jx < n_x ? Xs[jx] : EMPTY_KEY;
-			i32 h = -1;
-			for (i32 o = o0; o < o1; ++o) {
-				const bool takeq = i < n_s && (jx >= n_x || qh <= xh);
-				const u64 x = takeq ? qh : xh;
-				if (takeq) {
-					++i;
-					qh = i < n_s ? w.Q[i] : EMPTY_KEY;
-				} else {
-					++jx;
-					xh = jx < n_x ? Xs[jx] : EMPTY_KEY;
-				}
-				const u64 dg = key_diag(x);
-				bool keep = dg != prev_dg;
-				prev_dg = dg;
-				if (keep) {
-					if (h < 0) {  // first interval ending after dg
-						i32 a = 0, c = nh2;
-						while (a < c) {
-							const i32 mid = (a + c) >> 1;
-							if (Hh[mid] <= dg) a = mid + 1;
-							else c = mid;
-						}
-						h = a;
-					} else {
-						while (h < nh2 && Hh[h] <= dg) ++h;
-					}
-					keep = !(h < nh2 && Hl[h] <= dg);  // coalesced intervals: disjoint, sorted
-				}
-				if (keep) M[o0 + kept++] = x;  // kept states packed at the front of the range
-			}
-		}
-		// one scan of the per-thread counts; each thread copies its own kept states
-		const i32 at = b.scan(kept, &n_new);
-		for (i32 e = 0; e < kept; ++e) w.P[at + e] = M[o0 + e];
+		const bool q_lds = x_lds && used + n_s <= avail;
+		if (q_lds) stage(w.sort_lds + used, w.Q, n_s, b);
 		b.sync();
+		if (q_lds) n_new = merge_filter<NW>((lds_cptr)(w.sort_lds + used), n_s, (lds_cptr)Xs, n_x, Hl, Hh, nh2, M, w.P, b);
+		else if (x_lds) n_new = merge_filter<NW>((const u64 *)w.Q, n_s, (lds_cptr)Xs, n_x, Hl, Hh, nh2, M, w.P, b);
+		else n_new = merge_filter<NW>((const u64 *)w.Q, n_s, Xs, n_x, Hl, Hh, nh2, M, w.P, b);
 	} else {
 		// ---- P6: merge the sorted X into Q in place (merge path, right to left; Q first on ties)
 		if (n_x > 0) {
@@ -1129,10 +1206,9 @@ __device__ __forceinline__ u64 uni64(u64 x) {
 	       (u64)(u32)__builtin_amdgcn_readfirstlane((i32)(u32)(x >> 32)) << 32;
 }
 
-// Largest power-of-two run that the LDS arena of capacities lc can sort once
+// Largest power-of-two run that an LDS arena of le elements can sort once
 // a read has spilled (the arena is free then).
-__device__ __forceinline__ i32 arena_sort_cap(const Caps &lc) {
-	const size_t le = lds_elems(lc);
+__device__ __forceinline__ i32 arena_sort_cap(size_t le) {
 	i32 sc = 2;
 	while ((size_t)sc * 2 <= le && sc < (1 << 14)) sc <<= 1;
 	return sc;
@@ -1150,7 +1226,7 @@ template <int NW>
 __global__ __launch_bounds__(WAVE *NW) __attribute__((amdgpu_waves_per_eu(5))) void gwf_persistent_kernel(
     DGraph g, DReads rd, const i32 *order,
                                                                  const i32 *bin_off, i32 bin, i32 *heads, Caps lc,
-                                                                 Caps hc, i32 mw, u64 *scratch, u64 ws_elems_per_wg,
+                                                                 Caps hc, i32 mw, i32 arena_n, u64 *scratch, u64 ws_elems_per_wg,
                                                                  i32 *out_ed, i32 *out_status, int64_t *out_counts,
                                                                  GwfBeat *beats) {
 	extern __shared__ u64 arena[];
@@ -1164,8 +1240,8 @@ __global__ __launch_bounds__(WAVE *NW) __attribute__((amdgpu_waves_per_eu(5))) v
 	Bufs hb = carve(scratch + (size_t)blockIdx.x * ws_elems_per_wg, hc, true);
 	const i32 q_lo = bin_off[bin], q_n = bin_off[bin + 1] - q_lo;
 	hb.sort_lds = arena;  // the LDS arena is free once a read has spilled
-	hb.sort_cap = arena_sort_cap(lc);
-	hb.sort_n = (i32)lds_elems(lc);
+	hb.sort_cap = arena_sort_cap((size_t)arena_n);
+	hb.sort_n = arena_n;
 	hb.mw = mw;
 	GwfBeat *beat = beats ? beats + blockIdx.x : nullptr;
 	for (;;) {
@@ -1213,7 +1289,7 @@ __global__ __launch_bounds__(WAVE *NW) void gwf_serve_kernel(SrvArgs a) {
 	Blk<NW> b{red, redu};
 	Bufs lb = carve(arena, a.lc, false);
 	lb.mw = a.mw;
-	const i32 sort_cap = arena_sort_cap(a.lc);
+	const i32 sort_cap = arena_sort_cap(lds_elems(a.lc));
 	GwfBeat *beat = a.beat ? a.beat + blockIdx.x : nullptr;
 	i32 idle = 0;
 	u64 claimed_seen = 0;  // thread 0: a lower bound of the claim counter
@@ -1426,7 +1502,8 @@ static hipError_t launch_persist(const DGraph &g, const DReads &r, const i32 *or
                                  i32 *heads, const PersistCfg &cfg, u64 *scratch, size_t ws_bytes, i32 *out_ed,
                                  i32 *out_status, int64_t *out_counts, GwfBeat *beat, hipStream_t st) {
 	hipLaunchKernelGGL(gwf_persistent_kernel<NW>, dim3(cfg.wgs), dim3(WAVE * NW), cfg.lds_bytes, st, g, r, order,
-	                   bin_off, bin, heads, cfg.lds, cfg.hbm, cfg.mw, scratch, (u64)(ws_bytes / 8), out_ed, out_status,
+	                   bin_off, bin, heads, cfg.lds, cfg.hbm, cfg.mw, (i32)(cfg.lds_bytes / 8), scratch, (u64)(ws_bytes / 8),
+	                   out_ed, out_status,
 	                   out_counts, beat);
 	return hipGetLastError();
 }
@@ -1467,6 +1544,19 @@ static int occ(size_t lds_bytes) {
 	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, gwf_persistent_kernel<NW>, WAVE * NW, lds_bytes) != hipSuccess)
 		return 1;
 	return nb;
+}
+
+// The largest dynamic LDS size (a multiple of 256 bytes, at least lds_bytes)
+// that keeps per_cu workgroups of nw waves resident per CU, leaving `margin`
+// bytes of each workgroup's share unallocated.
+size_t gwf_persist_grow_lds(int nw, size_t lds_bytes, int per_cu, size_t margin) {
+	if (per_cu < 1) return lds_bytes;
+	size_t share = (160 * 1024) / (size_t)per_cu;
+	if (share <= margin) return lds_bytes;
+	size_t cand = (share - margin) / 256 * 256;
+	if (cand > gwf_max_lds_bytes()) cand = gwf_max_lds_bytes() / 256 * 256;
+	while (cand > lds_bytes && gwf_persist_occupancy(nw, cand) < per_cu) cand -= 256;
+	return cand > lds_bytes ? cand : lds_bytes;
 }
 
 int gwf_persist_occupancy(int nw, size_t lds_bytes) {

@@ -192,6 +192,11 @@ PersistCfg make_cfg(const RunCtx &cx, i32 nw, i32 m, i32 mw, i32 max_ql, bool en
 	c.lds_bytes = gwf_lds_bytes(c.lds);
 	c.hbm = caps_for(max_ql, 1);
 	const int per_cu = gwf_persist_occupancy(nw, c.lds_bytes);
+	// LDS left over at this occupancy becomes staging room for spilled reads.
+	// The occupancy query accepts sizes at which MI355X keeps one workgroup
+	// per CU out (2 waves x 10 per CU: 13.3 KB dynamic measured 9 resident,
+	// 12 KB 10), hence the 4 KB margin on each workgroup's share.
+	c.lds_bytes = gwf_persist_grow_lds(nw, c.lds_bytes, per_cu, (size_t)env_int("GWF_LDS_MARGIN", 4096));
 	c.wgs = std::max(1, per_cu) * std::max(1, cx.n_cu);
 	if (env_override) c.wgs = (i32)env_int("GWF_PERSIST_WGS", c.wgs);
 	return c;
