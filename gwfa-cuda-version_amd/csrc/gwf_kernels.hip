@@ -49,28 +49,57 @@ __device__ __forceinline__ u32 rank_below(u64 mask) {
 	return __builtin_amdgcn_mbcnt_hi((u32)(mask >> 32), __builtin_amdgcn_mbcnt_lo((u32)mask, 0u));
 }
 
+// Wave prefix operations on DPP lane moves (no LDS traffic): shifts by 1,
+// 2, 4, 8 within each row of 16 lanes, then row 0's last lane into row 1,
+// row 2's into row 3, and lane 31 into rows 2 and 3.  Lanes a move has no
+// source for receive `old` (the operation's identity).
+#define DPP_ROW_SHR(n) (0x110 + (n))
+#define DPP_ROW_BCAST15 0x142
+#define DPP_ROW_BCAST31 0x143
+#define DPP_WAVE_SHR1 0x138
+
+template <int CTRL, int ROWS>
+__device__ __forceinline__ i32 dpp_i32(i32 old, i32 v) {
+	return __builtin_amdgcn_update_dpp(old, v, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ u64 dpp_u64(u64 v) {
+	const u32 lo = (u32)dpp_i32<CTRL, ROWS>(0, (i32)(u32)v);
+	const u32 hi = (u32)dpp_i32<CTRL, ROWS>(0, (i32)(u32)(v >> 32));
+	return (u64)hi << 32 | lo;
+}
+
 // exclusive prefix sum over the wave; *total receives the wave sum
 __device__ __forceinline__ i32 wave_scan(i32 x, i32 *total) {
-	const u32 lane = lane_id();
-	i32 incl = x;
-#pragma unroll
-	for (int o = 1; o < WAVE; o <<= 1) {
-		const i32 y = __shfl_up(incl, o);
-		if (lane >= (u32)o) incl += y;
-	}
-	*total = __shfl(incl, WAVE - 1);
-	return incl - x;
+	i32 v = x;
+	v += dpp_i32<DPP_ROW_SHR(1), 0xf>(0, v);
+	v += dpp_i32<DPP_ROW_SHR(2), 0xf>(0, v);
+	v += dpp_i32<DPP_ROW_SHR(4), 0xf>(0, v);
+	v += dpp_i32<DPP_ROW_SHR(8), 0xf>(0, v);
+	v += dpp_i32<DPP_ROW_BCAST15, 0xa>(0, v);
+	v += dpp_i32<DPP_ROW_BCAST31, 0xc>(0, v);
+	*total = __builtin_amdgcn_readlane(v, WAVE - 1);
+	return v - x;
 }
+
+__device__ __forceinline__ u64 umax64(u64 a, u64 b) { return a > b ? a : b; }
 
 // inclusive prefix max over the wave (u64)
 __device__ __forceinline__ u64 wave_scan_max(u64 x) {
-	const u32 lane = lane_id();
-#pragma unroll
-	for (int o = 1; o < WAVE; o <<= 1) {
-		const u64 y = __shfl_up(x, o);
-		if (lane >= (u32)o) x = x > y ? x : y;
-	}
+	x = umax64(x, dpp_u64<DPP_ROW_SHR(1), 0xf>(x));
+	x = umax64(x, dpp_u64<DPP_ROW_SHR(2), 0xf>(x));
+	x = umax64(x, dpp_u64<DPP_ROW_SHR(4), 0xf>(x));
+	x = umax64(x, dpp_u64<DPP_ROW_SHR(8), 0xf>(x));
+	x = umax64(x, dpp_u64<DPP_ROW_BCAST15, 0xa>(x));
+	x = umax64(x, dpp_u64<DPP_ROW_BCAST31, 0xc>(x));
 	return x;
+}
+
+// lane l's value, l uniform over the wave
+__device__ __forceinline__ i32 lane_bcast(i32 v, i32 l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ u64 lane_bcast64(u64 v, i32 l) {
+	return (u64)(u32)__builtin_amdgcn_readlane((i32)(u32)v, l) |
+	       (u64)(u32)__builtin_amdgcn_readlane((i32)(u32)(v >> 32), l) << 32;
 }
 
 // ------------------------------------------------------------ block helpers
@@ -124,12 +153,15 @@ struct Blk {
 		if constexpr (NW == 1) return __ballot(p) != 0;
 		else return __syncthreads_or(p) != 0;
 	}
+	// block max of non-negative values
 	__device__ __forceinline__ i32 max(i32 x) const {
-#pragma unroll
-		for (int o = 32; o > 0; o >>= 1) {
-			const i32 y = __shfl_xor(x, o);
-			x = x > y ? x : y;
-		}
+		x = ::max(x, dpp_i32<DPP_ROW_SHR(1), 0xf>(0, x));
+		x = ::max(x, dpp_i32<DPP_ROW_SHR(2), 0xf>(0, x));
+		x = ::max(x, dpp_i32<DPP_ROW_SHR(4), 0xf>(0, x));
+		x = ::max(x, dpp_i32<DPP_ROW_SHR(8), 0xf>(0, x));
+		x = ::max(x, dpp_i32<DPP_ROW_BCAST15, 0xa>(0, x));
+		x = ::max(x, dpp_i32<DPP_ROW_BCAST31, 0xc>(0, x));
+		x = lane_bcast(x, WAVE - 1);
 		if constexpr (NW == 1) {
 			return x;
 		} else {
@@ -146,11 +178,10 @@ struct Blk {
 	// (0 for the first thread), *tot = block max
 	__device__ __forceinline__ u64 scan_max2(u64 x, u64 *excl, u64 *tot) const {
 		const u64 wi = wave_scan_max(x);
-		u64 we = __shfl_up(wi, 1);
-		if (lane_id() == 0) we = 0;
+		const u64 we = dpp_u64<DPP_WAVE_SHR1, 0xf>(wi);  // lane 0: 0
 		if constexpr (NW == 1) {
 			*excl = we;
-			*tot = __shfl(wi, WAVE - 1);
+			*tot = lane_bcast64(wi, WAVE - 1);
 			return wi;
 		} else {
 			if (lane_id() == WAVE - 1) redu[wid()] = wi;
@@ -233,8 +264,8 @@ __device__ __forceinline__ i32 extend_wave(bool act, const uint8_t *seq, u32 seq
 	while (lm) {  // cooperative extension of the long runs, one state at a time
 		const int src = __ffsll((unsigned long long)lm) - 1;
 		lm &= lm - 1;
-		const i32 kk = __shfl(k, src), mk_ = __shfl(max_k, src), dd = __shfl(d, src);
-		const u32 so = (u32)__shfl((i32)seq_off, src);
+		const i32 kk = lane_bcast(k, src), mk_ = lane_bcast(max_k, src), dd = lane_bcast(d, src);
+		const u32 so = (u32)lane_bcast((i32)seq_off, src);
 		const uint8_t *ts = seq + so;
 		const uint8_t *qd = q + dd;
 		i32 base = kk, res;
@@ -250,7 +281,7 @@ __device__ __forceinline__ i32 extend_wave(bool act, const uint8_t *seq, u32 seq
 			const u64 sm = __ballot(stop);
 			if (sm) {
 				const int f = __ffsll((unsigned long long)sm) - 1;
-				const i32 got = __shfl(my + m, f);
+				const i32 got = lane_bcast(my + m, f);
 				res = base + (got < avail ? got : avail);
 				break;
 			}
