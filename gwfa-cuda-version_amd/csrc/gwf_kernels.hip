@@ -359,21 +359,82 @@ __device__ __forceinline__ void sort_hbm(u64 *p, i32 n, u64 *lds, i32 lc, u64 *t
 	}
 }
 
+// The value of lane (this lane ^ S): quad permutations for S = 1, 2,
+// swizzles within 32 lanes for S = 4, 8, 16, a permute for S = 32.
+template <int S>
+__device__ __forceinline__ u32 xor_lane32(u32 v) {
+	if constexpr (S == 1) return (u32)dpp_i32<0xB1, 0xf>(0, (i32)v);  // quad_perm [1,0,3,2]
+	else if constexpr (S == 2) return (u32)dpp_i32<0x4E, 0xf>(0, (i32)v);  // quad_perm [2,3,0,1]
+	else if constexpr (S < 32) return (u32)__builtin_amdgcn_ds_swizzle((i32)v, 0x1f | (S << 10));
+	else return (u32)__shfl_xor((i32)v, 32);
+}
+template <int S>
+__device__ __forceinline__ u64 xor_lane(u64 x) {
+	return (u64)xor_lane32<S>((u32)(x >> 32)) << 32 | xor_lane32<S>((u32)x);
+}
+
+template <int SIZE, int STRIDE>
+__device__ __forceinline__ u64 bitonic_stage(u64 x, u32 lane) {
+	const u64 y = xor_lane<STRIDE>(x);
+	const bool up = (lane & SIZE) == 0;
+	const bool lower = (lane & STRIDE) == 0;
+	const u64 lo = x < y ? x : y, hi = x < y ? y : x;
+	return (lower == up) ? lo : hi;
+}
+template <int SIZE, int STRIDE>
+__device__ __forceinline__ u64 bitonic_merge_stages(u64 x, u32 lane) {
+	x = bitonic_stage<SIZE, STRIDE>(x, lane);
+	if constexpr (STRIDE > 1) x = bitonic_merge_stages<SIZE, STRIDE / 2>(x, lane);
+	return x;
+}
+
 // Bitonic sort of 64 keys held one per lane (ascending by lane).
 __device__ __forceinline__ u64 wave_sort64(u64 x) {
 	const u32 lane = lane_id();
-#pragma unroll
-	for (int size = 2; size <= WAVE; size <<= 1) {
-#pragma unroll
-		for (int stride = size >> 1; stride > 0; stride >>= 1) {
-			const u64 y = __shfl_xor(x, stride);
-			const bool up = (lane & size) == 0;
-			const bool lower = (lane & stride) == 0;
-			const u64 lo = x < y ? x : y, hi = x < y ? y : x;
-			x = (lower == up) ? lo : hi;
+	x = bitonic_merge_stages<2, 1>(x, lane);
+	x = bitonic_merge_stages<4, 2>(x, lane);
+	x = bitonic_merge_stages<8, 4>(x, lane);
+	x = bitonic_merge_stages<16, 8>(x, lane);
+	x = bitonic_merge_stages<32, 16>(x, lane);
+	x = bitonic_merge_stages<64, 32>(x, lane);
+	return x;
+}
+
+// One round of pairwise merges of the sorted runs of width wdt in src[0..n)
+// into dst (left run first on ties).  Thread t writes outputs [t E, t E + E)
+// (E a power of two dividing 2 wdt, so a range never crosses a pair): one
+// merge-path search, then a sequential merge from the two run heads.
+template <int NW>
+__device__ __forceinline__ void merge_round(const u64 *src, u64 *dst, i32 n, i32 wdt, const Blk<NW> &b) {
+	constexpr i32 NT = Blk<NW>::NT;
+	i32 E = 1;
+	while (E * NT < n && E < 2 * wdt) E <<= 1;
+	for (i32 o0 = (i32)b.tid() * E; o0 < n; o0 += NT * E) {
+		const i32 l = o0 / (2 * wdt) * (2 * wdt);
+		const i32 mid = l + wdt < n ? l + wdt : n, r = l + 2 * wdt < n ? l + 2 * wdt : n;
+		const u64 *A = src + l, *C = src + mid;
+		const i32 na = mid - l, nc = r - mid, dg = o0 - l;
+		i32 lo = dg - nc > 0 ? dg - nc : 0, hi = dg < na ? dg : na;
+		while (lo < hi) {
+			const i32 m = (lo + hi) >> 1;
+			if (A[m] <= C[dg - 1 - m]) lo = m + 1;
+			else hi = m;
+		}
+		i32 i = lo, j = dg - lo;
+		u64 ah = i < na ? A[i] : EMPTY_KEY, ch = j < nc ? C[j] : EMPTY_KEY;
+		const i32 oe = o0 + E < r ? o0 + E : r;
+		for (i32 o = o0; o < oe; ++o) {
+			const bool ta = i < na && (j >= nc || ah <= ch);
+			dst[o] = ta ? ah : ch;
+			if (ta) {
+				++i;
+				ah = i < na ? A[i] : EMPTY_KEY;
+			} else {
+				++j;
+				ch = j < nc ? C[j] : EMPTY_KEY;
+			}
 		}
 	}
-	return x;
 }
 
 // Sort src[0..n) into dst: every wave sorts runs of 64 in registers (no
@@ -412,11 +473,8 @@ __device__ __forceinline__ void sort_runs(u64 *src, i32 n, u64 *dst, i32 mw, con
 	}
 	u64 *a = src, *c = dst;
 	for (i32 wdt = WAVE; wdt < n; wdt <<= 1) {
-		for (i32 l = 0; l < n; l += 2 * wdt) {
-			const i32 mid = l + wdt < n ? l + wdt : n;
-			const i32 r = l + 2 * wdt < n ? l + 2 * wdt : n;
-			merge_keys(a + l, mid - l, a + mid, r - mid, c + l, b);
-		}
+		merge_round(a, c, n, wdt, b);
+		b.sync();
 		u64 *t = a;
 		a = c;
 		c = t;

@@ -216,7 +216,7 @@ gwf_tune_t builtin_tune() {
 	const i32 nw[] = {1, 2, 4, 4, 4};
 	const i32 m[] = {256, 512, 1024, 1024, 1024};
 	t.n_bins = 5;
-	for (int b = 0; b < 5; ++b) t.bin[b] = {est[b], nw[b], m[b], 8};
+	for (int b = 0; b < 5; ++b) t.bin[b] = {est[b], nw[b], m[b], 0};
 	return t;
 }
 
@@ -936,9 +936,9 @@ int gwf_autotune(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t max_evals, gwf_tune
 			return t;
 		};
 		const gwf_tune_t base = builtin_tune();
-		static const gwf_tune_bin_t cands[] = {{0, 4, 1024, 8}, {0, 4, 2048, 8}, {0, 4, 512, 8}, {0, 4, 256, 8},
-		                                       {0, 2, 512, 8},  {0, 2, 256, 8},  {0, 2, 128, 8}, {0, 1, 256, 8},
-		                                       {0, 1, 128, 8}};
+		static const gwf_tune_bin_t cands[] = {{0, 4, 1024, 0}, {0, 4, 2048, 0}, {0, 4, 512, 0}, {0, 4, 256, 0},
+		                                       {0, 2, 512, 0},  {0, 2, 256, 0},  {0, 2, 128, 0}, {0, 1, 256, 0},
+		                                       {0, 1, 128, 0}};
 		gwf_tune_t one;  // a single bin: one queue, one launch
 		memset(&one, 0, sizeof(one));
 		one.n_bins = 1;
@@ -972,7 +972,7 @@ int gwf_autotune(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t max_evals, gwf_tune
 			if (again >= 0 && one_again >= 0 && again < one_again * keep) best = again, best_t = cur;
 		}
 		// 3. the sort variant, for the whole table
-		static const i32 mws[] = {0, 2, 4, 16, 32};
+		static const i32 mws[] = {2, 4, 8};
 		for (i32 mw : mws) {
 			gwf_tune_t t = best_t;
 			for (int b = 0; b < t.n_bins; ++b) t.bin[b].merge_runs = mw;

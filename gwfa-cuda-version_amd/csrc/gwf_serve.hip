@@ -535,7 +535,7 @@ int gwf_serve_start(gwf_dgraph_t *dg, const gwf_serve_opts_t *opts, gwf_serve_t 
 	}
 	c.wgs = o.wgs > 0 ? o.wgs : std::max(1, gwf_serve_occupancy(c.nw, c.lds_bytes)) * std::max(1, n_cu);
 	c.hbm = gwf_caps_for(1, 1);  // unused: each request brings its own workspace capacities
-	c.mw = 8;
+	c.mw = 0;
 	if (e == hipSuccess) e = gwf_kernels_init();
 	if (e == hipSuccess) e = hipStreamCreate(&s->stream);
 	if (e == hipSuccess) e = host_coherent(&s->ctl_h, &s->ctl_d, 1);
