@@ -105,11 +105,15 @@ __device__ __forceinline__ u64 lane_bcast64(u64 v, i32 l) {
 // ------------------------------------------------------------ block helpers
 // Collectives over the NW waves of a workgroup.  Every thread must call them
 // the same number of times (trip counts of loops around them are uniform).
+// The per-wave partials alternate between two LDS slots, so one barrier per
+// collective suffices: a slot is written again only two collectives later,
+// after a barrier that every reader of its previous contents has passed.
 template <int NW>
 struct Blk {
 	static constexpr int NT = WAVE * NW;
-	i32 *red;   // LDS scratch, NW ints
-	u64 *redu;  // LDS scratch, NW u64
+	i32 *red;   // LDS scratch, 2 NW ints
+	u64 *redu;  // LDS scratch, 2 NW u64
+	mutable i32 par = 0;  // slot of the next collective
 	__device__ __forceinline__ u32 tid() const { return threadIdx.x; }
 	__device__ __forceinline__ u32 wid() const { return NW == 1 ? 0u : threadIdx.x / WAVE; }
 	__device__ __forceinline__ void sync() const { __syncthreads(); }
@@ -119,16 +123,17 @@ struct Blk {
 			*tot = c;
 			return 0;
 		} else {
-			if (lane_id() == 0) red[wid()] = c;
+			i32 *r = red + par * NW;
+			par ^= 1;
+			if (lane_id() == 0) r[wid()] = c;
 			sync();
 			i32 pre = 0, t = 0;
 #pragma unroll
 			for (int w = 0; w < NW; ++w) {
-				const i32 v = red[w];
+				const i32 v = r[w];
 				pre += w < (int)wid() ? v : 0;
 				t += v;
 			}
-			sync();
 			*tot = t;
 			return pre;
 		}
@@ -140,6 +145,16 @@ struct Blk {
 		*tot = t;
 		return p + pre;
 	}
+	// two exclusive scans in one (per-thread values below 256, block totals
+	// below 2^16): *pc receives the second, *ta / *tc the totals
+	__device__ __forceinline__ i32 scan2(i32 a, i32 c, i32 *pc, i32 *ta, i32 *tc) const {
+		i32 t;
+		const i32 p = scan(a | c << 16, &t);
+		*pc = p >> 16;
+		*ta = t & 0xffff;
+		*tc = t >> 16;
+		return p & 0xffff;
+	}
 	// compaction slot of a predicate; *n advances by the block count
 	__device__ __forceinline__ i32 slot(bool pred, i32 *n) const {
 		const u64 m = __ballot(pred);
@@ -148,6 +163,17 @@ struct Blk {
 		const i32 s = *n + pre + (i32)rank_below(m);
 		*n += t;
 		return s;
+	}
+	// slots of two predicates with one collective (*s2 receives the second)
+	__device__ __forceinline__ i32 slot2(bool p1, bool p2, i32 *n1, i32 *n2, i32 *s2) const {
+		const u64 m1 = __ballot(p1), m2 = __ballot(p2);
+		i32 t;
+		const i32 pre = wave_prefix(__popcll(m1) | __popcll(m2) << 16, &t);
+		const i32 s1 = *n1 + (pre & 0xffff) + (i32)rank_below(m1);
+		*s2 = *n2 + (pre >> 16) + (i32)rank_below(m2);
+		*n1 += t & 0xffff;
+		*n2 += t >> 16;
+		return s1;
 	}
 	__device__ __forceinline__ bool any(bool p) const {
 		if constexpr (NW == 1) return __ballot(p) != 0;
@@ -165,12 +191,13 @@ struct Blk {
 		if constexpr (NW == 1) {
 			return x;
 		} else {
-			if (lane_id() == 0) red[wid()] = x;
+			i32 *r = red + par * NW;
+			par ^= 1;
+			if (lane_id() == 0) r[wid()] = x;
 			sync();
-			i32 m = red[0];
+			i32 m = r[0];
 #pragma unroll
-			for (int w = 1; w < NW; ++w) m = m > red[w] ? m : red[w];
-			sync();
+			for (int w = 1; w < NW; ++w) m = m > r[w] ? m : r[w];
 			return m;
 		}
 	}
@@ -184,16 +211,17 @@ struct Blk {
 			*tot = lane_bcast64(wi, WAVE - 1);
 			return wi;
 		} else {
-			if (lane_id() == WAVE - 1) redu[wid()] = wi;
+			u64 *r = redu + par * NW;
+			par ^= 1;
+			if (lane_id() == WAVE - 1) r[wid()] = wi;
 			sync();
 			u64 pre = 0, t = 0;
 #pragma unroll
 			for (int w = 0; w < NW; ++w) {
-				const u64 v = redu[w];
+				const u64 v = r[w];
 				if (w < (int)wid()) pre = pre > v ? pre : v;
 				t = t > v ? t : v;
 			}
-			sync();
 			*excl = we > pre ? we : pre;
 			*tot = t;
 			return wi > pre ? wi : pre;
@@ -895,8 +923,8 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 			cs += (i32)sc[s].s0 + (i32)sc[s].s1 + (i32)sc[s].s2;
 			ct += (i32)sc[s].t0 + (i32)sc[s].t1 + (i32)sc[s].t2;
 		}
-		i32 tot;
-		i32 p = n_s + b.scan(cs, &tot);
+		i32 tot, pt, tot_t;
+		i32 p = n_s + b.scan2(cs, ct, &pt, &tot, &tot_t);
 		if (n_s + tot > cap.m) ovf = true;
 		else {
 #pragma unroll
@@ -909,7 +937,8 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 			}
 		}
 		n_s += tot;
-		p = n_t + b.scan(ct, &tot);
+		tot = tot_t;
+		p = n_t + pt;
 		if (n_t + tot > cap.t) ovf = true;
 		else {
 #pragma unroll
@@ -958,13 +987,14 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 				n_ext += m;
 				const bool absent = has && !qe && vis_insert(vis, (u64)wv << 32 | (u32)(i + 1));
 				const bool pa = absent && m;
-				const i32 sa = b.slot(pa, &n_na);
-				if (pa && sa < cap.f) w.F1[sa] = mk(wv, i + 1 - ol, ol);
 				const bool px = absent && !m;
 				const bool pq = has && qe;
-				i32 tx;
-				const i32 sx = n_x + b.scan(px ? 2 : pq ? 1 : 0, &tx);
+				i32 sx, ta, tx;
+				const i32 sa = n_na + b.scan2(pa ? 1 : 0, px ? 2 : pq ? 1 : 0, &sx, &ta, &tx);
+				sx += n_x;
+				n_na += ta;
 				n_x += tx;
+				if (pa && sa < cap.f) w.F1[sa] = mk(wv, i + 1 - ol, ol);
 				if (px && sx + 1 < cap.x) {
 					w.X[sx] = mk(wv, i - ol, ol);
 					w.X[sx + 1] = mk(wv, i + 1 - ol, ol);
@@ -976,9 +1006,9 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 			// this step join the history too (gwf-ed.cu:1177-1183); with
 			// v1 < 0 they change no distance and are left out
 			const bool t2 = rec && !qe && v1 >= 0;
-			const i32 st2 = b.slot(t2, &n_t);
+			i32 si;
+			const i32 st2 = b.slot2(t2, pi, &n_t, &n_x, &si);
 			if (t2 && st2 < cap.t) w.T[st2] = diag_id(v, d);
-			const i32 si = b.slot(pi, &n_x);
 			if (pi && si < cap.x) w.X[si] = mk(v, d + 1, k);
 		}
 		if (n_na > cap.f || n_x > cap.x) return STEP_SPILL;
@@ -1002,11 +1032,12 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 			const bool vend = act && !fnd && k == vl - 1;
 			const bool mid = act && !qend && !vend;
 			const bool qmid = act && qend && !fnd && k + 1 < vl;  // query end mid-vertex: deletion only (gwf-ed.cu:1237-1241)
-			const i32 sf = b.slot(vend, &n_f2);
-			if (vend && sf < cap.f) w.F0[sf] = mk(v, d, k);
-			i32 tx;
-			const i32 sx = n_x + b.scan(mid ? 3 : qmid ? 1 : 0, &tx);
+			i32 sx, tf, tx;
+			const i32 sf = n_f2 + b.scan2(vend ? 1 : 0, mid ? 3 : qmid ? 1 : 0, &sx, &tf, &tx);
+			sx += n_x;
+			n_f2 += tf;
 			n_x += tx;
+			if (vend && sf < cap.f) w.F0[sf] = mk(v, d, k);
 			if (mid && sx + 2 < cap.x) {
 				w.X[sx] = mk(v, d - 1, k + 1);
 				w.X[sx + 1] = mk(v, d, k + 1);
@@ -1275,8 +1306,8 @@ __global__ __launch_bounds__(WAVE) void gwf_plain_kernel(DGraph g, DReads rd, co
                                                          i32 *out_status, int64_t *out_counts) {
 	__shared__ u64 lds_sort[SORT_CHUNK];
 	__shared__ u64 vis[VIS_SLOTS];
-	__shared__ i32 red[1];
-	__shared__ u64 redu[1];
+	__shared__ i32 red[2];
+	__shared__ u64 redu[2];
 	const i32 blk = blockIdx.x;
 	if (blk >= n) return;
 	const i32 r = read_idx ? read_idx[blk] : blk;
@@ -1320,8 +1351,8 @@ __global__ __launch_bounds__(WAVE *NW) __attribute__((amdgpu_waves_per_eu(5))) v
                                                                  GwfBeat *beats) {
 	extern __shared__ u64 arena[];
 	__shared__ u64 vis[VIS_SLOTS];
-	__shared__ i32 red[NW];
-	__shared__ u64 redu[NW];
+	__shared__ i32 red[2 * NW];
+	__shared__ u64 redu[2 * NW];
 	__shared__ i32 next_q;
 	Blk<NW> b{red, redu};
 	Bufs lb = carve(arena, lc, false);
@@ -1368,8 +1399,8 @@ template <int NW>
 __global__ __launch_bounds__(WAVE *NW) void gwf_serve_kernel(SrvArgs a) {
 	extern __shared__ u64 arena[];
 	__shared__ u64 vis[VIS_SLOTS];
-	__shared__ i32 red[NW];
-	__shared__ u64 redu[NW];
+	__shared__ i32 red[2 * NW];
+	__shared__ u64 redu[2 * NW];
 	__shared__ int64_t s_seq, s_claim;
 	__shared__ i32 s_polls;
 	__shared__ SrvEntry s_e;
