@@ -709,7 +709,14 @@ __device__ __forceinline__ i32 merge_filter(QP Q, i32 n_s, XP X, i32 n_x, const 
 	// one scan of the per-thread counts; each thread copies its own kept states
 	i32 n_new;
 	const i32 at = b.scan(kept, &n_new);
-	for (i32 e = 0; e < kept; ++e) P[at + e] = M[o0 + e];
+	for (i32 e0 = 0; e0 < kept; e0 += 4) {
+		u64 t[4];
+#pragma unroll
+		for (i32 r = 0; r < 4; ++r) t[r] = e0 + r < kept ? M[o0 + e0 + r] : 0;
+#pragma unroll
+		for (i32 r = 0; r < 4; ++r)
+			if (e0 + r < kept) P[at + e0 + r] = t[r];
+	}
 	b.sync();
 	return n_new;
 }
