@@ -57,6 +57,7 @@ __device__ __forceinline__ u32 rank_below(u64 mask) {
 #define DPP_ROW_BCAST15 0x142
 #define DPP_ROW_BCAST31 0x143
 #define DPP_WAVE_SHR1 0x138
+#define DPP_WAVE_SHL1 0x130
 
 template <int CTRL, int ROWS>
 __device__ __forceinline__ i32 dpp_i32(i32 old, i32 v) {
@@ -830,136 +831,153 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 	const i32 tid = (i32)b.tid();
 	const Caps &cap = w.c;
 
-	// ---- P1: extend, detect the end, collect vertex ends (the closure's frontier).
-	// Each thread takes S1 states per round (strided, so loads coalesce) and
-	// issues the loads of all of them before using any: one chain of
-	// dependent loads (key, vertex record, first 8 bytes) per round instead
-	// of one per state.  States still matching after 8 bytes continue in
-	// extend_wave (16-byte lane steps, then 512-byte cooperative wave steps).
-	constexpr i32 S1 = 2;
-	bool found = false;
-	i32 n_f = 0;
-	for (i32 base = 0; base < n_a; base += NT * S1) {
-		u64 key[S1];
-		uint2 vi[S1];
+	// ---- P1+P2: extend every state in place, detect the end, collect vertex
+	// ends (the closure's frontier), and emit the in-vertex successors into Q
+	// (sorted) and their exits into T.  Each thread takes S consecutive states
+	// per round and issues the loads of all of them before using any: one
+	// chain of dependent loads (key, vertex record, first 8 bytes) per round.
+	// States still matching after 8 bytes continue in extend_wave (16-byte
+	// lane steps, then 512-byte cooperative wave steps).  Successors need the
+	// extended neighbours on both sides, so they are emitted one round behind,
+	// from registers: neighbours come from the adjacent lanes (DPP wave
+	// shifts), across waves and rounds from per-round edge slots (three rounds
+	// deep) kept in the visited-set buffer, which the closure clears later.
+	constexpr i32 S = 2;
+	const i32 RS = NT * S;
+	const i32 R = (n_a + RS - 1) / RS;
+	const i32 wv = (i32)b.wid();
+	const u32 lane = lane_id();
+	u64 *edgeF = vis, *edgeL = vis + 3 * NW;  // [3][NW]: first / last state of each wave's range
+	bool found = false, ovf = false;
+	i32 n_f = 0, n_s = 0, n_t = 0;
+	u64 pk[S];   // the previous round's extended states (EMPTY_KEY past the end)
+	i32 pvl[S];  // and their vertex lengths
 #pragma unroll
-		for (i32 s = 0; s < S1; ++s) {
-			const i32 j = base + s * NT + tid;
-			key[s] = j < n_a ? w.P[j] : EMPTY_KEY;
-		}
+	for (i32 s = 0; s < S; ++s) pk[s] = EMPTY_KEY, pvl[s] = 0;
+	for (i32 r = 0; r <= R; ++r) {
+		u64 nk[S];
+		i32 nvl[S];
 #pragma unroll
-		for (i32 s = 0; s < S1; ++s)
-			vi[s] = key[s] != EMPTY_KEY ? *(const uint2 *)&g.vinfo[key_v(key[s])] : make_uint2(0, 0);
-		i32 kk[S1], mk8[S1];
-		u64 x8[S1];
+		for (i32 s = 0; s < S; ++s) nk[s] = EMPTY_KEY, nvl[s] = 0;
+		if (r < R) {  // extend round r
+			const i32 j0 = r * RS + tid * S;
+			u64 key[S];
+			uint2 vi[S];
 #pragma unroll
-		for (i32 s = 0; s < S1; ++s) {
-			const i32 d = key_d(key[s]), vl = (i32)vi[s].y;
-			kk[s] = key_k(key[s]);
-			mk8[s] = (ql - d < vl ? ql - d : vl) - 1;
-			const bool go = key[s] != EMPTY_KEY && mk8[s] > kk[s];
-			x8[s] = go ? load8u(g.seq + vi[s].x + kk[s] + 1) ^ load8u(q + d + kk[s] + 1) : 0;
-		}
-		bool more[S1];
+			for (i32 s = 0; s < S; ++s) key[s] = j0 + s < n_a ? w.P[j0 + s] : EMPTY_KEY;
 #pragma unroll
-		for (i32 s = 0; s < S1; ++s) {
-			more[s] = false;
-			const i32 avail = mk8[s] - kk[s];
-			if (key[s] != EMPTY_KEY && avail > 0) {
-				const i32 m = x8[s] ? (i32)(__builtin_ctzll(x8[s]) >> 3) : 8;
-				if (m >= avail) { chars += avail; kk[s] = mk8[s]; }
-				else if (m < 8) { chars += m + 1; kk[s] += m; }
-				else { chars += 8; kk[s] += 8; more[s] = true; }
+			for (i32 s = 0; s < S; ++s)
+				vi[s] = key[s] != EMPTY_KEY ? *(const uint2 *)&g.vinfo[key_v(key[s])] : make_uint2(0, 0);
+			i32 kk[S], mk8[S];
+			u64 x8[S];
+#pragma unroll
+			for (i32 s = 0; s < S; ++s) {
+				const i32 d = key_d(key[s]), vl = (i32)vi[s].y;
+				kk[s] = key_k(key[s]);
+				mk8[s] = (ql - d < vl ? ql - d : vl) - 1;
+				const bool go = key[s] != EMPTY_KEY && mk8[s] > kk[s];
+				x8[s] = go ? load8u(g.seq + vi[s].x + kk[s] + 1) ^ load8u(q + d + kk[s] + 1) : 0;
 			}
-		}
+			bool more[S];
 #pragma unroll
-		for (i32 s = 0; s < S1; ++s)  // every thread calls it S1 times (it is a wave collective)
-			kk[s] = extend_wave(more[s], g.seq, vi[s].x, q, key_d(key[s]), kk[s], mk8[s], &chars);
-		bool vend[S1];
-		i32 nv = 0;
-#pragma unroll
-		for (i32 s = 0; s < S1; ++s) {
-			const bool act = key[s] != EMPTY_KEY;
-			const u32 v = key_v(key[s]);
-			const i32 d = key_d(key[s]), k = kk[s], vl = (i32)vi[s].y;
-			if (act) w.P[base + s * NT + tid] = mk(v, d, k);
-			const bool qend = act && d + k == ql - 1;
-			// the end (gwf-ed.cu:1231-1236): anywhere, or at the last base of v1
-			const bool fnd = qend && (v1 < 0 || (v == (u32)v1 && k == vl - 1));
-			found |= fnd;
-			// vertex ends go to the closure; with v1 >= 0 that includes query
-			// ends, which continue by deletion into the next vertices
-			vend[s] = act && !fnd && k == vl - 1;
-			nv += vend[s];
-		}
-		i32 tot;
-		i32 pos = n_f + b.scan(nv, &tot);
-#pragma unroll
-		for (i32 s = 0; s < S1; ++s)
-			if (vend[s]) {
-				if (pos < cap.f) w.F0[pos] = mk(key_v(key[s]), key_d(key[s]), kk[s]);
-				++pos;
+			for (i32 s = 0; s < S; ++s) {
+				more[s] = false;
+				const i32 avail = mk8[s] - kk[s];
+				if (key[s] != EMPTY_KEY && avail > 0) {
+					const i32 m = x8[s] ? (i32)(__builtin_ctzll(x8[s]) >> 3) : 8;
+					if (m >= avail) { chars += avail; kk[s] = mk8[s]; }
+					else if (m < 8) { chars += m + 1; kk[s] += m; }
+					else { chars += 8; kk[s] += 8; more[s] = true; }
+				}
 			}
-		n_f += tot;
+#pragma unroll
+			for (i32 s = 0; s < S; ++s)  // every thread calls it S times (it is a wave collective)
+				kk[s] = extend_wave(more[s], g.seq, vi[s].x, q, key_d(key[s]), kk[s], mk8[s], &chars);
+			bool vend[S];
+			i32 nv = 0;
+#pragma unroll
+			for (i32 s = 0; s < S; ++s) {
+				const bool act = key[s] != EMPTY_KEY;
+				const u32 v = key_v(key[s]);
+				const i32 d = key_d(key[s]), k = kk[s], vl = (i32)vi[s].y;
+				if (act) {
+					nk[s] = mk(v, d, k);
+					nvl[s] = vl;
+					// LDS arrays: keep the extension, so a spilled step restarts from it
+					if constexpr (!HBM) w.P[j0 + s] = nk[s];
+				}
+				const bool qend = act && d + k == ql - 1;
+				// the end (gwf-ed.cu:1231-1236): anywhere, or at the last base of v1
+				const bool fnd = qend && (v1 < 0 || (v == (u32)v1 && k == vl - 1));
+				found |= fnd;
+				// vertex ends go to the closure; with v1 >= 0 that includes query
+				// ends, which continue by deletion into the next vertices
+				vend[s] = act && !fnd && k == vl - 1;
+				nv += vend[s];
+			}
+			if (lane == 0) edgeF[(r % 3) * NW + wv] = nk[0];
+			if (lane == WAVE - 1) edgeL[(r % 3) * NW + wv] = nk[S - 1];
+			i32 tot;
+			i32 pos = n_f + b.scan(nv, &tot);  // (its barrier also publishes the edge slots)
+#pragma unroll
+			for (i32 s = 0; s < S; ++s)
+				if (vend[s]) {
+					if (pos < cap.f) w.F0[pos] = nk[s];
+					++pos;
+				}
+			n_f += tot;
+		}
+		if (r > 0) {  // successors of round r - 1
+			u64 c[S + 2];  // c[0] = left neighbour, c[1..S] = pk, c[S + 1] = right neighbour
+			c[0] = dpp_u64<DPP_WAVE_SHR1, 0xf>(pk[S - 1]);
+			c[S + 1] = dpp_u64<DPP_WAVE_SHL1, 0xf>(pk[0]);
+			if (lane == 0)
+				c[0] = wv > 0 ? edgeL[((r - 1) % 3) * NW + wv - 1] : r >= 2 ? edgeL[((r - 2) % 3) * NW + NW - 1] : EMPTY_KEY;
+			if (lane == WAVE - 1)
+				c[S + 1] = wv < NW - 1 ? edgeF[((r - 1) % 3) * NW + wv + 1] : r < R ? edgeF[(r % 3) * NW] : EMPTY_KEY;
+#pragma unroll
+			for (i32 s = 0; s < S; ++s) c[s + 1] = pk[s];
+			i32 cs = 0, ct = 0;
+			Succ sc[S];
+#pragma unroll
+			for (i32 s = 0; s < S; ++s) {
+				sc[s] = successors(pk[s] != EMPTY_KEY, c[s + 1], c[s], c[s + 2], pvl[s], ql);
+				cs += (i32)sc[s].s0 + (i32)sc[s].s1 + (i32)sc[s].s2;
+				ct += (i32)sc[s].t0 + (i32)sc[s].t1 + (i32)sc[s].t2;
+			}
+			i32 tot, pt, tot_t;
+			i32 p = n_s + b.scan2(cs, ct, &pt, &tot, &tot_t);
+			if (n_s + tot > cap.m) ovf = true;
+			else {
+#pragma unroll
+				for (i32 s = 0; s < S; ++s) {
+					const u32 v = key_v(c[s + 1]);
+					const i32 d = key_d(c[s + 1]);
+					if (sc[s].s0) w.Q[p++] = mk(v, d - 1, sc[s].k0);
+					if (sc[s].s1) w.Q[p++] = mk(v, d, sc[s].k1);
+					if (sc[s].s2) w.Q[p++] = mk(v, d + 1, sc[s].k2);
+				}
+			}
+			n_s += tot;
+			p = n_t + pt;
+			if (n_t + tot_t > cap.t) ovf = true;
+			else {
+#pragma unroll
+				for (i32 s = 0; s < S; ++s) {
+					const u32 v = key_v(c[s + 1]);
+					const i32 d = key_d(c[s + 1]);
+					if (sc[s].t0) w.T[p++] = diag_id(v, d - 1);
+					if (sc[s].t1) w.T[p++] = diag_id(v, d);
+					if (sc[s].t2) w.T[p++] = diag_id(v, d + 1);
+				}
+			}
+			n_t += tot_t;
+		}
+#pragma unroll
+		for (i32 s = 0; s < S; ++s) pk[s] = nk[s], pvl[s] = nvl[s];
 	}
 	if (b.any(found)) return STEP_FOUND;
-	if (n_f > cap.f) return STEP_SPILL;
-
-	// ---- P2: in-vertex successors into Q (sorted) and their exits into T.
-	// Each thread takes S2 consecutive states per round (plus one neighbour on
-	// each side), so the outputs, written in thread order, stay sorted.
-	constexpr i32 S2 = 2;
-	i32 n_s = 0, n_t = 0;
-	bool ovf = false;
-	for (i32 base = 0; base < n_a; base += NT * S2) {
-		const i32 j0 = base + tid * S2;
-		u64 c[S2 + 2];  // c[0] = P[j0 - 1] ... c[S2 + 1] = P[j0 + S2]
-#pragma unroll
-		for (i32 s = 0; s < S2 + 2; ++s) {
-			const i32 j = j0 - 1 + s;
-			c[s] = (j >= 0 && j < n_a && j < base + NT * S2 + 1) ? w.P[j] : EMPTY_KEY;
-		}
-		i32 vl[S2];
-#pragma unroll
-		for (i32 s = 0; s < S2; ++s) vl[s] = (j0 + s < n_a) ? (i32)g.vinfo[key_v(c[s + 1])].y : 0;
-		i32 cs = 0, ct = 0;
-		Succ sc[S2];
-#pragma unroll
-		for (i32 s = 0; s < S2; ++s) {
-			sc[s] = successors(j0 + s < n_a, c[s + 1], c[s], c[s + 2], vl[s], ql);
-			cs += (i32)sc[s].s0 + (i32)sc[s].s1 + (i32)sc[s].s2;
-			ct += (i32)sc[s].t0 + (i32)sc[s].t1 + (i32)sc[s].t2;
-		}
-		i32 tot, pt, tot_t;
-		i32 p = n_s + b.scan2(cs, ct, &pt, &tot, &tot_t);
-		if (n_s + tot > cap.m) ovf = true;
-		else {
-#pragma unroll
-			for (i32 s = 0; s < S2; ++s) {
-				const u32 v = key_v(c[s + 1]);
-				const i32 d = key_d(c[s + 1]);
-				if (sc[s].s0) w.Q[p++] = mk(v, d - 1, sc[s].k0);
-				if (sc[s].s1) w.Q[p++] = mk(v, d, sc[s].k1);
-				if (sc[s].s2) w.Q[p++] = mk(v, d + 1, sc[s].k2);
-			}
-		}
-		n_s += tot;
-		tot = tot_t;
-		p = n_t + pt;
-		if (n_t + tot > cap.t) ovf = true;
-		else {
-#pragma unroll
-			for (i32 s = 0; s < S2; ++s) {
-				const u32 v = key_v(c[s + 1]);
-				const i32 d = key_d(c[s + 1]);
-				if (sc[s].t0) w.T[p++] = diag_id(v, d - 1);
-				if (sc[s].t1) w.T[p++] = diag_id(v, d);
-				if (sc[s].t2) w.T[p++] = diag_id(v, d + 1);
-			}
-		}
-		n_t += tot;
-	}
-	if (ovf) return STEP_SPILL;  // (ovf is uniform: derived from block totals)
+	if (n_f > cap.f || ovf) return STEP_SPILL;  // (ovf is uniform: derived from block totals)
 	const i32 n_t1 = n_t;  // exits of in-vertex successors (sorted, unique)
 
 	// ---- P3: vertex-crossing closure
