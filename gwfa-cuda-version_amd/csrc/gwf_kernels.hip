@@ -230,13 +230,16 @@ struct Blk {
 	}
 };
 
-// unaligned 8-byte load from a buffer padded past its end
+// unaligned 8-byte load from a buffer padded past its end: one 16-byte load
+// from the enclosing 8-byte boundary, then a funnel shift.  (Global loads
+// need only dword alignment; the vector type keeps the two halves one
+// instruction.)
+typedef u64 u64x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ u64 load8u(const uint8_t *p) {
 	const u32 mis = (u32)((uintptr_t)p & 7);
-	const u64 *w = (const u64 *)(p - mis);  // pointer arithmetic keeps the address space
+	const u64x2 w = *(const u64x2 *)(p - mis);  // pointer arithmetic keeps the address space
 	const u32 sh = mis * 8;
-	const u64 lo = w[0], hi = w[1];
-	return sh ? (lo >> sh) | (hi << (64 - sh)) : lo;
+	return sh ? (w.x >> sh) | (w.y << (64 - sh)) : w.x;
 }
 
 // first index in a[0..n) with a[i] >= x (lower) or a[i] > x (upper)
