@@ -647,24 +647,62 @@ __device__ __forceinline__ void stage(u64 *dst, const u64 *src, i32 n, const Blk
 	}
 }
 
+// One step of the fused merge below: takes the next state of Q / X (Q
+// first on ties) into x and returns whether it is kept (first of its
+// diagonal, outside the history intervals).  The cursors live in the caller.
+template <typename QP, typename XP>
+__device__ __forceinline__ bool mf_next(QP Q, i32 n_s, XP X, i32 n_x, const u64 *Hl, const u64 *Hh, i32 nh, i32 &i,
+                                        i32 &jx, u64 &qh, u64 &xh, u64 &prev_dg, i32 &h, u64 &x) {
+	const bool takeq = i < n_s && (jx >= n_x || qh <= xh);
+	x = takeq ? qh : xh;
+	if (takeq) {
+		++i;
+		qh = i < n_s ? Q[i] : EMPTY_KEY;
+	} else {
+		++jx;
+		xh = jx < n_x ? X[jx] : EMPTY_KEY;
+	}
+	const u64 dg = key_diag(x);
+	bool keep = dg != prev_dg;
+	prev_dg = dg;
+	if (keep) {
+		if (h < 0) {  // first interval ending after dg
+			i32 a = 0, c = nh;
+			while (a < c) {
+				const i32 mid = (a + c) >> 1;
+				if (Hh[mid] <= dg) a = mid + 1;
+				else c = mid;
+			}
+			h = a;
+		} else {
+			while (h < nh && Hh[h] <= dg) ++h;
+		}
+		keep = !(h < nh && Hl[h] <= dg);  // coalesced intervals: disjoint, sorted
+	}
+	return keep;
+}
+
 // Merge of the sorted successor lists Q (n_s) and X (n_x) fused with the
 // furthest-state-per-diagonal rule and the history test (P6+P7 for arrays
 // in HBM).  Thread t takes the contiguous range [t E, t E + E) of the merged
 // order: one merge-path search gives its start in Q and X; it then merges
 // sequentially (Q first on ties), keeps the first (furthest) state of each
 // diagonal and tests the history (coalesced intervals Hl/Hh, nh of them)
-// with a pointer that only moves forward.  Kept states are packed at the
-// front of each range in M, then compacted into P; returns their count.
+// with a pointer that only moves forward.  Kept states go to P: straight from
+// registers when E <= 8, else packed at the front of each range in M and
+// copied after the scan.  Returns their count.
 // QP and XP are lds_cptr or const u64 * (where each list lives).
 template <int NW, typename QP, typename XP>
 __device__ __forceinline__ i32 merge_filter(QP Q, i32 n_s, XP X, i32 n_x, const u64 *Hl, const u64 *Hh, i32 nh, u64 *M,
                                             u64 *P, const Blk<NW> &b) {
 	constexpr i32 NT = Blk<NW>::NT;
+	constexpr i32 CH = 8;
 	const i32 n_m = n_s + n_x;
 	const i32 E = (n_m + NT - 1) / NT;
 	const i32 o0 = (i32)b.tid() * E;
 	const i32 o1 = o0 + E < n_m ? o0 + E : n_m;
-	i32 kept = 0;
+	i32 i = 0, jx = 0, h = -1;
+	u64 prev_dg = ~0ull, qh = EMPTY_KEY, xh = EMPTY_KEY;
 	if (o0 < n_m) {
 		i32 lo = o0 - n_x > 0 ? o0 - n_x : 0, up = o0 < n_s ? o0 : n_s;
 		while (lo < up) {
@@ -672,54 +710,43 @@ __device__ __forceinline__ i32 merge_filter(QP Q, i32 n_s, XP X, i32 n_x, const 
 			if (Q[mid] <= X[o0 - 1 - mid]) lo = mid + 1;
 			else up = mid;
 		}
-		i32 i = lo, jx = o0 - lo;
-		u64 prev_dg = ~0ull;  // diag of the merged state before o0 (none: o0 == 0)
-		if (o0 > 0) {
+		i = lo, jx = o0 - lo;
+		if (o0 > 0) {  // diag of the merged state before o0 (none: o0 == 0)
 			const bool from_q = i > 0 && (jx == 0 || Q[i - 1] > X[jx - 1]);
 			prev_dg = key_diag(from_q ? Q[i - 1] : X[jx - 1]);
 		}
-		u64 qh = i < n_s ? Q[i] : EMPTY_KEY, xh = jx < n_x ? X[jx] : EMPTY_KEY;
-		i32 h = -1;
-		for (i32 o = o0; o < o1; ++o) {
-			const bool takeq = i < n_s && (jx >= n_x || qh <= xh);
-			const u64 x = takeq ? qh : xh;
-			if (takeq) {
-				++i;
-				qh = i < n_s ? Q[i] : EMPTY_KEY;
-			} else {
-				++jx;
-				xh = jx < n_x ? X[jx] : EMPTY_KEY;
-			}
-			const u64 dg = key_diag(x);
-			bool keep = dg != prev_dg;
-			prev_dg = dg;
-			if (keep) {
-				if (h < 0) {  // first interval ending after dg
-					i32 a = 0, c = nh;
-					while (a < c) {
-						const i32 mid = (a + c) >> 1;
-						if (Hh[mid] <= dg) a = mid + 1;
-						else c = mid;
-					}
-					h = a;
-				} else {
-					while (h < nh && Hh[h] <= dg) ++h;
-				}
-				keep = !(h < nh && Hl[h] <= dg);  // coalesced intervals: disjoint, sorted
-			}
-			if (keep) M[o0 + kept++] = x;
-		}
+		qh = i < n_s ? Q[i] : EMPTY_KEY, xh = jx < n_x ? X[jx] : EMPTY_KEY;
 	}
-	// one scan of the per-thread counts; each thread copies its own kept states
 	i32 n_new;
-	const i32 at = b.scan(kept, &n_new);
-	for (i32 e0 = 0; e0 < kept; e0 += 4) {
-		u64 t[4];
+	if (E <= CH) {
+		u64 kv[CH];
+		u32 km = 0;
 #pragma unroll
-		for (i32 r = 0; r < 4; ++r) t[r] = e0 + r < kept ? M[o0 + e0 + r] : 0;
+		for (i32 u = 0; u < CH; ++u) {
+			u64 x = 0;
+			if (o0 + u < o1 && mf_next(Q, n_s, X, n_x, Hl, Hh, nh, i, jx, qh, xh, prev_dg, h, x)) km |= 1u << u;
+			kv[u] = x;
+		}
+		const i32 at = b.scan(__popc(km), &n_new);
 #pragma unroll
-		for (i32 r = 0; r < 4; ++r)
-			if (e0 + r < kept) P[at + e0 + r] = t[r];
+		for (i32 u = 0; u < CH; ++u)
+			if ((km >> u) & 1) P[at + __popc(km & ((1u << u) - 1))] = kv[u];
+	} else {
+		i32 kept = 0;
+		for (i32 o = o0; o < o1; ++o) {
+			u64 x;
+			if (mf_next(Q, n_s, X, n_x, Hl, Hh, nh, i, jx, qh, xh, prev_dg, h, x)) M[o0 + kept++] = x;
+		}
+		// one scan of the per-thread counts; each thread copies its own kept states
+		const i32 at = b.scan(kept, &n_new);
+		for (i32 e0 = 0; e0 < kept; e0 += 4) {
+			u64 t[4];
+#pragma unroll
+			for (i32 r = 0; r < 4; ++r) t[r] = e0 + r < kept ? M[o0 + e0 + r] : 0;
+#pragma unroll
+			for (i32 r = 0; r < 4; ++r)
+				if (e0 + r < kept) P[at + e0 + r] = t[r];
+		}
 	}
 	b.sync();
 	return n_new;
