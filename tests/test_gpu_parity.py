@@ -182,6 +182,25 @@ def test_lds_spill_to_hbm(gpu, orc, monkeypatch):
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("nw,margin", [(1, 1 << 20), (2, 1 << 20), (2, 0)])
+def test_hbm_staging_fallbacks(gpu, orc, monkeypatch, nw, margin):
+    """Spilled reads with an LDS scratch too small for the closure
+    successors, the history merge, Q or the history (sorted and merged in
+    HBM, kept states compacted through the workspace), or as large as the
+    occupancy allows (margin 0: the arena may keep a workgroup per CU out,
+    which only delays it)."""
+    _say(f"case: persistent kernel nw={nw} lds m=64 arena margin={margin}, config 4 reads")
+    monkeypatch.setenv("GWF_PERSIST_NW", str(nw))
+    monkeypatch.setenv("GWF_PERSIST_M", "64")
+    monkeypatch.setenv("GWF_LDS_MARGIN", str(margin))
+    g, r = gpu.make_workload(4, 8)
+    g.index()
+    want, _, _ = orc.batch(g, r)
+    got, st = _gpu_eds(gpu, g, r, 1)
+    assert st["n_spill"] == len(want)
+    assert np.array_equal(got, want)
+
+
 @pytest.mark.parametrize("nw,m", [(1, 256), (2, 512), (4, 2048)])
 def test_persistent_configurations(gpu, orc, monkeypatch, nw, m):
     _say(f"case: persistent kernel nw={nw} lds m={m}")
