@@ -114,6 +114,7 @@ struct Blk {
 	static constexpr int NT = WAVE * NW;
 	i32 *red;   // LDS scratch, 2 NW ints
 	u64 *redu;  // LDS scratch, 2 NW u64
+	i32 *xch;   // LDS, 4 ints: results handed from one wave to the others
 	mutable i32 par = 0;  // slot of the next collective
 	__device__ __forceinline__ u32 tid() const { return threadIdx.x; }
 	__device__ __forceinline__ u32 wid() const { return NW == 1 ? 0u : threadIdx.x / WAVE; }
@@ -227,6 +228,60 @@ struct Blk {
 			*tot = t;
 			return wi > pre ? wi : pre;
 		}
+	}
+};
+
+// Collectives over one wave, with the Blk interface (lane ids as thread ids,
+// no LDS partials, no barriers): for work that one wave of a workgroup does
+// while the others do something else.
+struct WaveB {
+	static constexpr int NT = WAVE;
+	__device__ __forceinline__ u32 tid() const { return lane_id(); }
+	__device__ __forceinline__ u32 wid() const { return 0u; }
+	// LDS accesses of one wave are performed in order; this only keeps the
+	// compiler from moving memory accesses across the point
+	__device__ __forceinline__ void sync() const {
+		__builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+		__builtin_amdgcn_wave_barrier();
+	}
+	__device__ __forceinline__ i32 scan(i32 x, i32 *tot) const { return wave_scan(x, tot); }
+	__device__ __forceinline__ i32 scan2(i32 a, i32 c, i32 *pc, i32 *ta, i32 *tc) const {
+		i32 t;
+		const i32 p = wave_scan(a | c << 16, &t);
+		*pc = p >> 16;
+		*ta = t & 0xffff;
+		*tc = t >> 16;
+		return p & 0xffff;
+	}
+	__device__ __forceinline__ i32 slot(bool pred, i32 *n) const {
+		const u64 m = __ballot(pred);
+		const i32 s = *n + (i32)rank_below(m);
+		*n += __popcll(m);
+		return s;
+	}
+	__device__ __forceinline__ i32 slot2(bool p1, bool p2, i32 *n1, i32 *n2, i32 *s2) const {
+		const u64 m1 = __ballot(p1), m2 = __ballot(p2);
+		const i32 s1 = *n1 + (i32)rank_below(m1);
+		*s2 = *n2 + (i32)rank_below(m2);
+		*n1 += __popcll(m1);
+		*n2 += __popcll(m2);
+		return s1;
+	}
+	__device__ __forceinline__ bool any(bool p) const { return __ballot(p) != 0; }
+	__device__ __forceinline__ i32 max(i32 x) const {  // of non-negative values
+		x = ::max(x, dpp_i32<DPP_ROW_SHR(1), 0xf>(0, x));
+		x = ::max(x, dpp_i32<DPP_ROW_SHR(2), 0xf>(0, x));
+		x = ::max(x, dpp_i32<DPP_ROW_SHR(4), 0xf>(0, x));
+		x = ::max(x, dpp_i32<DPP_ROW_SHR(8), 0xf>(0, x));
+		x = ::max(x, dpp_i32<DPP_ROW_BCAST15, 0xa>(0, x));
+		x = ::max(x, dpp_i32<DPP_ROW_BCAST31, 0xc>(0, x));
+		return lane_bcast(x, WAVE - 1);
+	}
+	__device__ __forceinline__ u64 scan_max2(u64 x, u64 *excl, u64 *tot) const {
+		const u64 wi = wave_scan_max(x);
+		*excl = dpp_u64<DPP_WAVE_SHR1, 0xf>(wi);  // lane 0: 0
+		*tot = lane_bcast64(wi, WAVE - 1);
+		return wi;
 	}
 };
 
@@ -518,15 +573,15 @@ __device__ __forceinline__ void sort_runs(u64 *src, i32 n, u64 *dst, i32 mw, con
 }
 
 // Merge two interval lists sorted by lo (a before c on ties) into olo/ohi.
-template <int NW>
+template <class B>
 __device__ __forceinline__ void merge_intervals2(const u64 *alo, const u64 *ahi, i32 na, const u64 *clo,
-                                                 const u64 *chi, i32 nc, u64 *olo, u64 *ohi, const Blk<NW> &b) {
-	for (i32 j = (i32)b.tid(); j < na; j += Blk<NW>::NT) {
+                                                 const u64 *chi, i32 nc, u64 *olo, u64 *ohi, const B &b) {
+	for (i32 j = (i32)b.tid(); j < na; j += B::NT) {
 		const u64 x = alo[j];
 		const i32 p = j + lower_bound(clo, nc, x);
 		olo[p] = x, ohi[p] = ahi[j];
 	}
-	for (i32 t = (i32)b.tid(); t < nc; t += Blk<NW>::NT) {
+	for (i32 t = (i32)b.tid(); t < nc; t += B::NT) {
 		const u64 x = clo[t];
 		const i32 p = t + upper_bound(alo, na, x);
 		olo[p] = x, ohi[p] = chi[t];
@@ -571,10 +626,10 @@ __device__ __forceinline__ void merge_intervals_ranges(const u64 *alo, const u64
 
 // Runs of consecutive ids in the sorted, duplicate-free T[0..n) as half-open
 // intervals [lo, hi); returns their count.
-template <int NW>
-__device__ __forceinline__ i32 ti_runs(const u64 *T, i32 n, u64 *lo, u64 *hi, const Blk<NW> &b) {
+template <class B>
+__device__ __forceinline__ i32 ti_runs(const u64 *T, i32 n, u64 *lo, u64 *hi, const B &b) {
 	i32 n_ti = 0;
-	for (i32 base = 0; base < n; base += Blk<NW>::NT) {
+	for (i32 base = 0; base < n; base += B::NT) {
 		const i32 j = base + (i32)b.tid();
 		const bool act = j < n;
 		const u64 x = act ? T[j] : 0;
@@ -592,11 +647,11 @@ __device__ __forceinline__ i32 ti_runs(const u64 *T, i32 n, u64 *lo, u64 *hi, co
 // Coalesce overlapping or adjacent intervals of a list sorted by lo
 // (gwf_intv_merge_adj, gwf-ed.cu:266-296).  Returns the output count (the
 // caller guarantees room for n).
-template <int NW>
-__device__ __forceinline__ i32 coalesce(const u64 *lo, const u64 *hi, i32 n, u64 *olo, u64 *ohi, const Blk<NW> &b) {
+template <class B>
+__device__ __forceinline__ i32 coalesce(const u64 *lo, const u64 *hi, i32 n, u64 *olo, u64 *ohi, const B &b) {
 	u64 carry = 0;  // max hi over previous chunks
 	i32 ng = 0;     // groups started so far
-	for (i32 base = 0; base < n; base += Blk<NW>::NT) {
+	for (i32 base = 0; base < n; base += B::NT) {
 		const i32 j = base + (i32)b.tid();
 		const bool act = j < n;
 		const u64 l = act ? lo[j] : EMPTY_KEY;
@@ -629,9 +684,9 @@ __device__ __forceinline__ bool forbidden(const u64 *hlo, const u64 *hhi, i32 nh
 typedef const __attribute__((address_space(3))) u64 *lds_cptr;
 
 // dst[0..n) = src[0..n), four loads in flight per thread (no barrier).
-template <int NW>
-__device__ __forceinline__ void stage(u64 *dst, const u64 *src, i32 n, const Blk<NW> &b) {
-	constexpr i32 NT = Blk<NW>::NT;
+template <class B>
+__device__ __forceinline__ void stage(u64 *dst, const u64 *src, i32 n, const B &b) {
+	constexpr i32 NT = B::NT;
 	for (i32 j0 = (i32)b.tid(); j0 < n; j0 += 4 * NT) {
 		u64 v[4];
 #pragma unroll
@@ -849,6 +904,142 @@ __device__ __forceinline__ Succ successors(bool act, u64 cur, u64 prv, u64 nxt, 
 
 enum { STEP_NEXT = 0, STEP_FOUND = 1, STEP_EMPTY = 2, STEP_SPILL = 3 };
 
+// P5 for a spilled read, in the LDS scratch: exit runs of T[0..n_tt),
+// current history (n_h intervals), merged list; the new history goes to the
+// top of the scratch (2 (n_h + n_tt) elements reserved there), where it
+// stays for the filter, and to Hlo/Hhi in HBM for the next step.  Needs
+// 2 n_tt + 2 n_h + 4 (n_h + n_tt) <= sort_n.  Returns its interval count.
+template <class B>
+__device__ __forceinline__ i32 hist_lds(const Bufs &w, const B &b, const u64 *Hc_lo, const u64 *Hc_hi, i32 n_h, i32 n_tt,
+                                        u64 *Hlo, u64 *Hhi) {
+	const i32 cap_h = n_h + n_tt;
+	u64 *l = w.sort_lds;
+	u64 *hc = l + 2 * n_tt, *Clo = hc + 2 * n_h, *Chi = Clo + cap_h;
+	u64 *Olo = l + w.sort_n - 2 * cap_h, *Ohi = Olo + cap_h;
+	stage(hc, Hc_lo, n_h, b);
+	stage(hc + n_h, Hc_hi, n_h, b);
+	const i32 n_ti = ti_runs(w.T, n_tt, l, l + n_tt, b);
+	b.sync();
+	merge_intervals2(hc, hc + n_h, n_h, l, l + n_tt, n_ti, Clo, Chi, b);
+	const i32 nh2 = coalesce(Clo, Chi, n_h + n_ti, Olo, Ohi, b);
+	for (i32 t = (i32)b.tid(); t < nh2; t += B::NT) Hlo[t] = Olo[t], Hhi[t] = Ohi[t];
+	return nh2;
+}
+
+// P3, the vertex-crossing closure of one step, by frontier levels: a
+// frontier state at a vertex end follows each out-arc (one 16-byte record:
+// target, overlap, first base); a match enters the target and is extended
+// to its end (the next level) or to a mismatch, a mismatch gives the
+// deletion and mismatch successors, and a state that did not match every
+// arc an insertion.  A per-step visited set keeps each (vertex, query
+// position) once.  Appends successors to X and (v1 >= 0) exits to T;
+// returns STEP_NEXT, STEP_FOUND or STEP_SPILL.  B: the whole workgroup
+// (Blk) or one wave (WaveB).
+template <class B>
+__device__ __forceinline__ int closure(const DGraph &g, const uint8_t *q, i32 ql, const Bufs &w, const B &b, u64 *vis,
+                                       i32 v1, i32 n_f, i32 &n_x, i32 &n_t, int64_t &chars) {
+	constexpr i32 NT = B::NT;
+	const i32 tid = (i32)b.tid();
+	const Caps &cap = w.c;
+	bool found = false;
+	for (i32 t = tid; t < VIS_SLOTS; t += NT) vis[t] = EMPTY_KEY;
+	b.sync();
+	bool closure_origin = false;
+	while (n_f > 0) {
+		i32 n_na = 0;
+		for (i32 base = 0; base < n_f; base += NT) {
+			const i32 j = base + tid;
+			const bool act = j < n_f;
+			const u64 key = act ? w.F0[j] : 0;
+			const u32 v = key_v(key);
+			const i32 d = key_d(key), k = key_k(key);
+			const i32 i = d + k;
+			const uint4 vi = act ? g.vinfo[v] : make_uint4(0, 0, 0, 0);
+			const bool rec = act && closure_origin;
+			// query end at a vertex end (only with v1 >= 0): delete into every
+			// next vertex, nothing else (gwf-ed.cu:1244-1254)
+			const bool qe = act && i + 1 == ql;
+			const uint8_t c = act ? q[i + 1] : 0;
+			const i32 deg = act ? (i32)vi.w : 0;
+			const i32 maxdeg = b.max(deg);
+			i32 n_ext = 0;
+			for (i32 e = 0; e < maxdeg; ++e) {
+				const bool has = e < deg;
+				const uint4 ar = has ? g.arc[vi.z + e] : make_uint4(0, 0, 0, 0);
+				const u32 wv = ar.x;
+				const i32 ol = (i32)(ar.y & 0xFFFFFFu);
+				const bool m = has && !qe && (uint8_t)(ar.y >> 24) == c;
+				n_ext += m;
+				const bool absent = has && !qe && vis_insert(vis, (u64)wv << 32 | (u32)(i + 1));
+				const bool pa = absent && m;
+				const bool px = absent && !m;
+				const bool pq = has && qe;
+				i32 sx, ta, tx;
+				const i32 sa = n_na + b.scan2(pa ? 1 : 0, px ? 2 : pq ? 1 : 0, &sx, &ta, &tx);
+				sx += n_x;
+				n_na += ta;
+				n_x += tx;
+				if (pa && sa < cap.f) w.F1[sa] = mk(wv, i + 1 - ol, ol);
+				if (px && sx + 1 < cap.x) {
+					w.X[sx] = mk(wv, i - ol, ol);
+					w.X[sx + 1] = mk(wv, i + 1 - ol, ol);
+				}
+				if (pq && sx < cap.x) w.X[sx] = mk(wv, i - ol, ol);
+			}
+			const bool pi = rec && !qe && (deg == 0 || n_ext != deg);
+			// with v1 >= 0 the exits of states that entered their vertex in
+			// this step join the history too (gwf-ed.cu:1177-1183); with
+			// v1 < 0 they change no distance and are left out
+			const bool t2 = rec && !qe && v1 >= 0;
+			i32 si;
+			const i32 st2 = b.slot2(t2, pi, &n_t, &n_x, &si);
+			if (t2 && st2 < cap.t) w.T[st2] = diag_id(v, d);
+			if (pi && si < cap.x) w.X[si] = mk(v, d + 1, k);
+		}
+		if (n_na > cap.f || n_x > cap.x) return STEP_SPILL;
+		b.sync();
+		// extend the states that entered a vertex with a match
+		i32 n_f2 = 0;
+		for (i32 base = 0; base < n_na; base += NT) {
+			const i32 j = base + tid;
+			const bool act = j < n_na;
+			const u64 key = act ? w.F1[j] : 0;
+			const u32 v = key_v(key);
+			const i32 d = key_d(key);
+			i32 k = key_k(key);
+			const uint4 vi = act ? g.vinfo[v] : make_uint4(0, 0, 0, 0);
+			const i32 vl = (i32)vi.y;
+			const i32 max_k = (ql - d < vl ? ql - d : vl) - 1;
+			k = extend_wave(act, g.seq, vi.x, q, d, k, max_k, &chars);
+			const bool qend = act && d + k == ql - 1;
+			const bool fnd = qend && (v1 < 0 || (v == (u32)v1 && k == vl - 1));
+			found |= fnd;
+			const bool vend = act && !fnd && k == vl - 1;
+			const bool mid = act && !qend && !vend;
+			const bool qmid = act && qend && !fnd && k + 1 < vl;  // query end mid-vertex: deletion only (gwf-ed.cu:1237-1241)
+			i32 sx, tf, tx;
+			const i32 sf = n_f2 + b.scan2(vend ? 1 : 0, mid ? 3 : qmid ? 1 : 0, &sx, &tf, &tx);
+			sx += n_x;
+			n_f2 += tf;
+			n_x += tx;
+			if (vend && sf < cap.f) w.F0[sf] = mk(v, d, k);
+			if (mid && sx + 2 < cap.x) {
+				w.X[sx] = mk(v, d - 1, k + 1);
+				w.X[sx + 1] = mk(v, d, k + 1);
+				w.X[sx + 2] = mk(v, d + 1, k);
+			}
+			if (qmid && sx < cap.x) w.X[sx] = mk(v, d - 1, k + 1);
+		}
+		if (b.any(found)) return STEP_FOUND;
+		if (n_f2 > cap.f || n_x > cap.x) return STEP_SPILL;
+		b.sync();
+		n_f = n_f2;
+		closure_origin = true;
+	}
+
+	return STEP_NEXT;
+}
+
 // One score step.  Reads P[0..n_a) (the wavefront, sorted) and the history
 // H[hs] (n_h intervals); on STEP_NEXT leaves the next wavefront in P and the
 // updated history in H[hs ^ 1].  On STEP_SPILL nothing the next attempt needs
@@ -1010,101 +1201,43 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 	if (n_f > cap.f || ovf) return STEP_SPILL;  // (ovf is uniform: derived from block totals)
 	const i32 n_t1 = n_t;  // exits of in-vertex successors (sorted, unique)
 
-	// ---- P3: vertex-crossing closure
-	for (i32 t = tid; t < VIS_SLOTS; t += NT) vis[t] = EMPTY_KEY;
-	b.sync();
-	i32 n_x = 0;
-	bool closure_origin = false;
-	while (n_f > 0) {
-		i32 n_na = 0;
-		for (i32 base = 0; base < n_f; base += NT) {
-			const i32 j = base + tid;
-			const bool act = j < n_f;
-			const u64 key = act ? w.F0[j] : 0;
-			const u32 v = key_v(key);
-			const i32 d = key_d(key), k = key_k(key);
-			const i32 i = d + k;
-			const uint4 vi = act ? g.vinfo[v] : make_uint4(0, 0, 0, 0);
-			const bool rec = act && closure_origin;
-			// query end at a vertex end (only with v1 >= 0): delete into every
-			// next vertex, nothing else (gwf-ed.cu:1244-1254)
-			const bool qe = act && i + 1 == ql;
-			const uint8_t c = act ? q[i + 1] : 0;
-			const i32 deg = act ? (i32)vi.w : 0;
-			const i32 maxdeg = b.max(deg);
-			i32 n_ext = 0;
-			for (i32 e = 0; e < maxdeg; ++e) {
-				const bool has = e < deg;
-				const uint4 ar = has ? g.arc[vi.z + e] : make_uint4(0, 0, 0, 0);
-				const u32 wv = ar.x;
-				const i32 ol = (i32)(ar.y & 0xFFFFFFu);
-				const bool m = has && !qe && (uint8_t)(ar.y >> 24) == c;
-				n_ext += m;
-				const bool absent = has && !qe && vis_insert(vis, (u64)wv << 32 | (u32)(i + 1));
-				const bool pa = absent && m;
-				const bool px = absent && !m;
-				const bool pq = has && qe;
-				i32 sx, ta, tx;
-				const i32 sa = n_na + b.scan2(pa ? 1 : 0, px ? 2 : pq ? 1 : 0, &sx, &ta, &tx);
-				sx += n_x;
-				n_na += ta;
-				n_x += tx;
-				if (pa && sa < cap.f) w.F1[sa] = mk(wv, i + 1 - ol, ol);
-				if (px && sx + 1 < cap.x) {
-					w.X[sx] = mk(wv, i - ol, ol);
-					w.X[sx + 1] = mk(wv, i + 1 - ol, ol);
-				}
-				if (pq && sx < cap.x) w.X[sx] = mk(wv, i - ol, ol);
+	// ---- P3: vertex-crossing closure.  For a spilled read with v1 < 0 (the
+	// closure records no exits, so the history update needs only the exits
+	// of the in-vertex successors) and two waves, P5 runs beside it: wave 0
+	// follows the closure, wave 1 updates the history in the LDS scratch
+	// (with four waves both stay block-wide: measured faster).  A spill
+	// found by the closure then ends the read's attempt (HBM capacities are
+	// re-sized by the host), so P5 having run changes nothing.
+	u64 *Hcur_lo = hs ? w.H1lo : w.H0lo, *Hcur_hi = hs ? w.H1hi : w.H0hi;
+	u64 *Hlo = hs ? w.H0lo : w.H1lo, *Hhi = hs ? w.H0hi : w.H1hi;
+	i32 n_x = 0, nh2 = 0;
+	i32 hist_res = 0;                    // HBM mode: LDS elements reserved at the top of the scratch
+	const u64 *Hl_f = Hlo, *Hh_f = Hhi;  // where the filter reads the new history
+	bool p5_done = false;
+	if constexpr (HBM && NW == 2) {
+		const i32 cap_h = n_h + n_t;
+		if (v1 < 0 && 2 * n_t + 2 * n_h + 4 * cap_h <= w.sort_n && cap_h <= cap.h) {
+			const WaveB wb;
+			if (b.wid() == 0) {
+				i32 nx = 0, nt = n_t;
+				const int rc = closure(g, q, ql, w, wb, vis, v1, n_f, nx, nt, chars);
+				if (lane_id() == 0) b.xch[0] = rc, b.xch[1] = nx;
+			} else if (b.wid() == 1) {
+				const i32 h2 = hist_lds(w, wb, Hcur_lo, Hcur_hi, n_h, n_t, Hlo, Hhi);
+				if (lane_id() == 0) b.xch[2] = h2;
 			}
-			const bool pi = rec && !qe && (deg == 0 || n_ext != deg);
-			// with v1 >= 0 the exits of states that entered their vertex in
-			// this step join the history too (gwf-ed.cu:1177-1183); with
-			// v1 < 0 they change no distance and are left out
-			const bool t2 = rec && !qe && v1 >= 0;
-			i32 si;
-			const i32 st2 = b.slot2(t2, pi, &n_t, &n_x, &si);
-			if (t2 && st2 < cap.t) w.T[st2] = diag_id(v, d);
-			if (pi && si < cap.x) w.X[si] = mk(v, d + 1, k);
+			b.sync();
+			const int rc = b.xch[0];
+			n_x = b.xch[1], nh2 = b.xch[2];
+			if (rc != STEP_NEXT) return rc;
+			hist_res = 2 * cap_h;
+			Hl_f = w.sort_lds + w.sort_n - 2 * cap_h, Hh_f = Hl_f + cap_h;
+			p5_done = true;
 		}
-		if (n_na > cap.f || n_x > cap.x) return STEP_SPILL;
-		b.sync();
-		// extend the states that entered a vertex with a match
-		i32 n_f2 = 0;
-		for (i32 base = 0; base < n_na; base += NT) {
-			const i32 j = base + tid;
-			const bool act = j < n_na;
-			const u64 key = act ? w.F1[j] : 0;
-			const u32 v = key_v(key);
-			const i32 d = key_d(key);
-			i32 k = key_k(key);
-			const uint4 vi = act ? g.vinfo[v] : make_uint4(0, 0, 0, 0);
-			const i32 vl = (i32)vi.y;
-			const i32 max_k = (ql - d < vl ? ql - d : vl) - 1;
-			k = extend_wave(act, g.seq, vi.x, q, d, k, max_k, &chars);
-			const bool qend = act && d + k == ql - 1;
-			const bool fnd = qend && (v1 < 0 || (v == (u32)v1 && k == vl - 1));
-			found |= fnd;
-			const bool vend = act && !fnd && k == vl - 1;
-			const bool mid = act && !qend && !vend;
-			const bool qmid = act && qend && !fnd && k + 1 < vl;  // query end mid-vertex: deletion only (gwf-ed.cu:1237-1241)
-			i32 sx, tf, tx;
-			const i32 sf = n_f2 + b.scan2(vend ? 1 : 0, mid ? 3 : qmid ? 1 : 0, &sx, &tf, &tx);
-			sx += n_x;
-			n_f2 += tf;
-			n_x += tx;
-			if (vend && sf < cap.f) w.F0[sf] = mk(v, d, k);
-			if (mid && sx + 2 < cap.x) {
-				w.X[sx] = mk(v, d - 1, k + 1);
-				w.X[sx + 1] = mk(v, d, k + 1);
-				w.X[sx + 2] = mk(v, d + 1, k);
-			}
-			if (qmid && sx < cap.x) w.X[sx] = mk(v, d - 1, k + 1);
-		}
-		if (b.any(found)) return STEP_FOUND;
-		if (n_f2 > cap.f || n_x > cap.x) return STEP_SPILL;
-		b.sync();
-		n_f = n_f2;
-		closure_origin = true;
+	}
+	if (!p5_done) {
+		const int rc = closure(g, q, ql, w, b, vis, v1, n_f, n_x, n_t, chars);
+		if (rc != STEP_NEXT) return rc;
 	}
 
 	// ---- capacity checks for the destructive part of the step (P is free from here on)
@@ -1142,28 +1275,12 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 	}
 
 	// ---- P5: history H[hs ^ 1] = coalesce(H[hs] u runs of this step's exits)
-	u64 *Hcur_lo = hs ? w.H1lo : w.H0lo, *Hcur_hi = hs ? w.H1hi : w.H0hi;
-	u64 *Hlo = hs ? w.H0lo : w.H1lo, *Hhi = hs ? w.H0hi : w.H1hi;
-	i32 nh2;
-	i32 hist_res = 0;                    // HBM mode: LDS elements reserved at the top of the scratch
-	const u64 *Hl_f = Hlo, *Hh_f = Hhi;  // where the filter reads the new history
-	const i32 cap_h = n_h + n_tt;        // bound on the merged interval count
-	if (HBM && 2 * n_tt + 2 * n_h + 4 * cap_h <= w.sort_n) {
-		// in the LDS scratch: exit runs, current history, merged list; the new
-		// history at the top, where it stays for the filter (and is copied to
-		// HBM for the next step)
-		u64 *l = w.sort_lds;
-		u64 *hc = l + 2 * n_tt, *Clo = hc + 2 * n_h, *Chi = Clo + cap_h;
-		u64 *Olo = l + w.sort_n - 2 * cap_h, *Ohi = Olo + cap_h;
-		stage(hc, Hcur_lo, n_h, b);
-		stage(hc + n_h, Hcur_hi, n_h, b);
-		const i32 n_ti = ti_runs(w.T, n_tt, l, l + n_tt, b);
-		b.sync();
-		merge_intervals2(hc, hc + n_h, n_h, l, l + n_tt, n_ti, Clo, Chi, b);
-		nh2 = coalesce(Clo, Chi, n_h + n_ti, Olo, Ohi, b);
-		for (i32 t = tid; t < nh2; t += NT) Hlo[t] = Olo[t], Hhi[t] = Ohi[t];
+	const i32 cap_h = n_h + n_tt;  // bound on the merged interval count
+	if (p5_done) {
+	} else if (HBM && 2 * n_tt + 2 * n_h + 4 * cap_h <= w.sort_n) {
+		nh2 = hist_lds(w, b, Hcur_lo, Hcur_hi, n_h, n_tt, Hlo, Hhi);
 		hist_res = 2 * cap_h;
-		Hl_f = Olo, Hh_f = Ohi;
+		Hl_f = w.sort_lds + w.sort_n - 2 * cap_h, Hh_f = Hl_f + cap_h;
 	} else {
 		u64 *TIlo = w.P, *TIhi = w.P + n_tt;
 		const i32 n_ti = ti_runs(w.T, n_tt, TIlo, TIhi, b);
@@ -1363,10 +1480,11 @@ __global__ __launch_bounds__(WAVE) void gwf_plain_kernel(DGraph g, DReads rd, co
 	__shared__ u64 vis[VIS_SLOTS];
 	__shared__ i32 red[2];
 	__shared__ u64 redu[2];
+	__shared__ i32 xch[4];
 	const i32 blk = blockIdx.x;
 	if (blk >= n) return;
 	const i32 r = read_idx ? read_idx[blk] : blk;
-	Blk<1> b{red, redu};
+	Blk<1> b{red, redu, xch};
 	Bufs w = carve(scratch + (size_t)blk * ws_elems_per_read, caps, true);
 	w.sort_lds = lds_sort;
 	w.sort_cap = SORT_CHUNK;
@@ -1408,8 +1526,9 @@ __global__ __launch_bounds__(WAVE *NW) __attribute__((amdgpu_waves_per_eu(5))) v
 	__shared__ u64 vis[VIS_SLOTS];
 	__shared__ i32 red[2 * NW];
 	__shared__ u64 redu[2 * NW];
+	__shared__ i32 xch[4];
 	__shared__ i32 next_q;
-	Blk<NW> b{red, redu};
+	Blk<NW> b{red, redu, xch};
 	Bufs lb = carve(arena, lc, false);
 	lb.mw = mw;
 	Bufs hb = carve(scratch + (size_t)blockIdx.x * ws_elems_per_wg, hc, true);
@@ -1456,12 +1575,13 @@ __global__ __launch_bounds__(WAVE *NW) void gwf_serve_kernel(SrvArgs a) {
 	__shared__ u64 vis[VIS_SLOTS];
 	__shared__ i32 red[2 * NW];
 	__shared__ u64 redu[2 * NW];
+	__shared__ i32 xch[4];
 	__shared__ int64_t s_seq, s_claim;
 	__shared__ i32 s_polls;
 	__shared__ SrvEntry s_e;
 	constexpr i32 NT = Blk<NW>::NT;
 	static_assert(sizeof(SrvEntry) == 64, "SrvEntry is eight u64 words");
-	Blk<NW> b{red, redu};
+	Blk<NW> b{red, redu, xch};
 	Bufs lb = carve(arena, a.lc, false);
 	lb.mw = a.mw;
 	const i32 sort_cap = arena_sort_cap(lds_elems(a.lc));
