@@ -1213,6 +1213,7 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 	i32 n_x = 0, nh2 = 0;
 	i32 hist_res = 0;                    // HBM mode: LDS elements reserved at the top of the scratch
 	const u64 *Hl_f = Hlo, *Hh_f = Hhi;  // where the filter reads the new history
+	i32 q_res = 0;                       // HBM mode: Q staged in the LDS scratch just below the history
 	bool p5_done = false;
 	if constexpr (HBM && NW == 2) {
 		const i32 cap_h = n_h + n_t;
@@ -1224,6 +1225,8 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 				if (lane_id() == 0) b.xch[0] = rc, b.xch[1] = nx;
 			} else if (b.wid() == 1) {
 				const i32 h2 = hist_lds(w, wb, Hcur_lo, Hcur_hi, n_h, n_t, Hlo, Hhi);
+				// and Q, for the merge, right below the new history
+				if (n_s <= w.sort_n - 2 * cap_h) stage(w.sort_lds + w.sort_n - 2 * cap_h - n_s, w.Q, n_s, wb);
 				if (lane_id() == 0) b.xch[2] = h2;
 			}
 			b.sync();
@@ -1231,6 +1234,7 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 			n_x = b.xch[1], nh2 = b.xch[2];
 			if (rc != STEP_NEXT) return rc;
 			hist_res = 2 * cap_h;
+			q_res = n_s <= w.sort_n - 2 * cap_h ? n_s : 0;
 			Hl_f = w.sort_lds + w.sort_n - 2 * cap_h, Hh_f = Hl_f + cap_h;
 			p5_done = true;
 		}
@@ -1296,7 +1300,7 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 	i32 lds_used = 0;     // HBM mode: elements of the LDS scratch holding the sorted X
 	bool x_lds = false;  // HBM mode: the sorted X is in the LDS scratch
 	if constexpr (HBM) {
-		if (2 * n_x <= w.sort_n - hist_res) {  // stage X in LDS, sort it there and merge from there
+		if (2 * n_x <= w.sort_n - hist_res - q_res) {  // stage X in LDS, sort it there and merge from there
 			stage(w.sort_lds, w.X, n_x, b);
 			b.sync();
 			sort_runs(w.sort_lds, n_x, w.sort_lds + n_x, w.mw, b);
@@ -1306,7 +1310,7 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 		} else {
 			if (hist_res) {  // the sort takes the whole scratch: the filter reads the history from HBM
 				b.sync();
-				hist_res = 0;
+				hist_res = 0, q_res = 0;
 				Hl_f = Hlo, Hh_f = Hhi;
 			}
 			sort_hbm(w.X, n_x, w.sort_lds, w.sort_cap, w.sort_tmp, b);
@@ -1325,7 +1329,7 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 		u64 *M = w.F0;  // F0 and F1 are adjacent: 2 cap.f >= n_m
 		const u64 *Hl = Hl_f, *Hh = Hh_f;
 		i32 used = lds_used;
-		const i32 avail = w.sort_n - hist_res;
+		const i32 avail = w.sort_n - hist_res - q_res;
 		if (!hist_res && used + 2 * nh2 <= avail) {
 			u64 *hl = w.sort_lds + used, *hh = hl + nh2;
 			stage(hl, Hlo, nh2, b);
@@ -1333,10 +1337,11 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 			Hl = hl, Hh = hh;
 			used += 2 * nh2;
 		}
-		const bool q_lds = x_lds && used + n_s <= avail;
-		if (q_lds) stage(w.sort_lds + used, w.Q, n_s, b);
+		const bool q_lds = x_lds && (q_res || used + n_s <= avail);
+		u64 *Ql = q_res ? w.sort_lds + w.sort_n - hist_res - q_res : w.sort_lds + used;
+		if (q_lds && !q_res) stage(Ql, w.Q, n_s, b);
 		b.sync();
-		if (q_lds) n_new = merge_filter<NW>((lds_cptr)(w.sort_lds + used), n_s, (lds_cptr)Xs, n_x, Hl, Hh, nh2, M, w.P, b);
+		if (q_lds) n_new = merge_filter<NW>((lds_cptr)Ql, n_s, (lds_cptr)Xs, n_x, Hl, Hh, nh2, M, w.P, b);
 		else if (x_lds) n_new = merge_filter<NW>((const u64 *)w.Q, n_s, (lds_cptr)Xs, n_x, Hl, Hh, nh2, M, w.P, b);
 		else n_new = merge_filter<NW>((const u64 *)w.Q, n_s, Xs, n_x, Hl, Hh, nh2, M, w.P, b);
 	} else {
