@@ -463,10 +463,10 @@ __device__ __forceinline__ u64 xor_lane(u64 x) {
 template <int SIZE, int STRIDE>
 __device__ __forceinline__ u64 bitonic_stage(u64 x, u32 lane) {
 	const u64 y = xor_lane<STRIDE>(x);
-	const bool up = (lane & SIZE) == 0;
-	const bool lower = (lane & STRIDE) == 0;
-	const u64 lo = x < y ? x : y, hi = x < y ? y : x;
-	return (lower == up) ? lo : hi;
+	// the lower lane of an ascending pair (or the upper of a descending one)
+	// keeps the smaller key; the comparison and this choice combine as masks
+	const bool take_min = ((lane & SIZE) == 0) == ((lane & STRIDE) == 0);
+	return ((x < y) == take_min) ? x : y;
 }
 template <int SIZE, int STRIDE>
 __device__ __forceinline__ u64 bitonic_merge_stages(u64 x, u32 lane) {
