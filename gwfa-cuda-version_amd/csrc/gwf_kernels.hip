@@ -511,7 +511,7 @@ __device__ __forceinline__ void merge_round(const u64 *src, u64 *dst, i32 n, i32
 		u64 ah = i < na ? A[i] : EMPTY_KEY, ch = j < nc ? C[j] : EMPTY_KEY;
 		const i32 oe = o0 + E < r ? o0 + E : r;
 		for (i32 o = o0; o < oe; ++o) {
-			const bool ta = i < na && (j >= nc || ah <= ch);
+			const bool ta = ah <= ch;  // an exhausted run's head is EMPTY_KEY, above every key
 			dst[o] = ta ? ah : ch;
 			if (ta) {
 				++i;
@@ -704,11 +704,13 @@ __device__ __forceinline__ void stage(u64 *dst, const u64 *src, i32 n, const B &
 
 // One step of the fused merge below: takes the next state of Q / X (Q
 // first on ties) into x and returns whether it is kept (first of its
-// diagonal, outside the history intervals).  The cursors live in the caller.
+// diagonal, outside the history intervals).  The cursors live in the caller;
+// an exhausted list's head is EMPTY_KEY, above every key; h is the first
+// history interval ending after the previous kept diagonal.
 template <typename QP, typename XP>
 __device__ __forceinline__ bool mf_next(QP Q, i32 n_s, XP X, i32 n_x, const u64 *Hl, const u64 *Hh, i32 nh, i32 &i,
                                         i32 &jx, u64 &qh, u64 &xh, u64 &prev_dg, i32 &h, u64 &x) {
-	const bool takeq = i < n_s && (jx >= n_x || qh <= xh);
+	const bool takeq = qh <= xh;
 	x = takeq ? qh : xh;
 	if (takeq) {
 		++i;
@@ -721,17 +723,7 @@ __device__ __forceinline__ bool mf_next(QP Q, i32 n_s, XP X, i32 n_x, const u64 
 	bool keep = dg != prev_dg;
 	prev_dg = dg;
 	if (keep) {
-		if (h < 0) {  // first interval ending after dg
-			i32 a = 0, c = nh;
-			while (a < c) {
-				const i32 mid = (a + c) >> 1;
-				if (Hh[mid] <= dg) a = mid + 1;
-				else c = mid;
-			}
-			h = a;
-		} else {
-			while (h < nh && Hh[h] <= dg) ++h;
-		}
+		while (h < nh && Hh[h] <= dg) ++h;
 		keep = !(h < nh && Hl[h] <= dg);  // coalesced intervals: disjoint, sorted
 	}
 	return keep;
@@ -756,7 +748,7 @@ __device__ __forceinline__ i32 merge_filter(QP Q, i32 n_s, XP X, i32 n_x, const 
 	const i32 E = (n_m + NT - 1) / NT;
 	const i32 o0 = (i32)b.tid() * E;
 	const i32 o1 = o0 + E < n_m ? o0 + E : n_m;
-	i32 i = 0, jx = 0, h = -1;
+	i32 i = 0, jx = 0, h = 0;
 	u64 prev_dg = ~0ull, qh = EMPTY_KEY, xh = EMPTY_KEY;
 	if (o0 < n_m) {
 		i32 lo = o0 - n_x > 0 ? o0 - n_x : 0, up = o0 < n_s ? o0 : n_s;
@@ -771,6 +763,14 @@ __device__ __forceinline__ i32 merge_filter(QP Q, i32 n_s, XP X, i32 n_x, const 
 			prev_dg = key_diag(from_q ? Q[i - 1] : X[jx - 1]);
 		}
 		qh = i < n_s ? Q[i] : EMPTY_KEY, xh = jx < n_x ? X[jx] : EMPTY_KEY;
+		// the first history interval ending after this range's first diagonal
+		const u64 dg0 = key_diag(qh <= xh ? qh : xh);
+		i32 c = nh;
+		while (h < c) {
+			const i32 mid = (h + c) >> 1;
+			if (Hh[mid] <= dg0) h = mid + 1;
+			else c = mid;
+		}
 	}
 	i32 n_new;
 	if (E <= CH) {
