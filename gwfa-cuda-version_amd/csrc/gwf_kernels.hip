@@ -384,68 +384,6 @@ __device__ __forceinline__ i32 extend_wave(bool act, const uint8_t *seq, u32 seq
 }
 
 // ------------------------------------------------------------ sort / merge
-// Bitonic sort of a[0..N), N a power of two.
-template <int NW>
-__device__ __forceinline__ void bitonic(u64 *a, i32 N, const Blk<NW> &b) {
-	for (i32 size = 2; size <= N; size <<= 1) {
-		for (i32 stride = size >> 1; stride > 0; stride >>= 1) {
-			for (i32 t = b.tid(); t < (N >> 1); t += Blk<NW>::NT) {
-				const i32 i = 2 * t - (t & (stride - 1));
-				const i32 j = i + stride;
-				const bool up = (i & size) == 0;
-				const u64 x = a[i], y = a[j];
-				if ((x > y) == up) { a[i] = y; a[j] = x; }
-			}
-			b.sync();
-		}
-	}
-}
-
-// Merge sorted a[0..na) and b[0..nb) into out (a before b on ties).
-template <int NW>
-__device__ __forceinline__ void merge_keys(const u64 *a, i32 na, const u64 *c, i32 nc, u64 *out, const Blk<NW> &b) {
-	for (i32 j = b.tid(); j < na; j += Blk<NW>::NT) {
-		const u64 x = a[j];
-		out[j + lower_bound(c, nc, x)] = x;
-	}
-	for (i32 t = b.tid(); t < nc; t += Blk<NW>::NT) {
-		const u64 x = c[t];
-		out[t + upper_bound(a, na, x)] = x;
-	}
-	b.sync();
-}
-
-// Sort p[0..n) (HBM): runs of `lc` sorted in the LDS scratch, then pairwise
-// merges through tmp (n elements).
-template <int NW>
-__device__ __forceinline__ void sort_hbm(u64 *p, i32 n, u64 *lds, i32 lc, u64 *tmp, const Blk<NW> &b) {
-	if (n <= 1) return;
-	for (i32 c0 = 0; c0 < n; c0 += lc) {
-		const i32 m = n - c0 < lc ? n - c0 : lc;
-		const i32 N = next_pow2(m);
-		for (i32 t = b.tid(); t < N; t += Blk<NW>::NT) lds[t] = t < m ? p[c0 + t] : EMPTY_KEY;
-		b.sync();
-		bitonic(lds, N, b);
-		for (i32 t = b.tid(); t < m; t += Blk<NW>::NT) p[c0 + t] = lds[t];
-		b.sync();
-	}
-	u64 *src = p, *dst = tmp;
-	for (i32 w = lc; w < n; w <<= 1) {
-		for (i32 l = 0; l < n; l += 2 * w) {
-			const i32 mid = l + w < n ? l + w : n;
-			const i32 r = l + 2 * w < n ? l + 2 * w : n;
-			merge_keys(src + l, mid - l, src + mid, r - mid, dst + l, b);
-		}
-		u64 *t = src;
-		src = dst;
-		dst = t;
-	}
-	if (src != p) {
-		for (i32 t = b.tid(); t < n; t += Blk<NW>::NT) p[t] = src[t];
-		b.sync();
-	}
-}
-
 // The value of lane (this lane ^ S): quad permutations for S = 1, 2,
 // swizzles within 32 lanes for S = 4, 8, 16, a permute for S = 32.
 template <int S>
@@ -568,6 +506,35 @@ __device__ __forceinline__ void sort_runs(u64 *src, i32 n, u64 *dst, i32 mw, con
 	}
 	if (a != dst) {
 		for (i32 j = (i32)b.tid(); j < n; j += Blk<NW>::NT) dst[j] = a[j];
+		b.sync();
+	}
+}
+
+// Sort p[0..n) (HBM): chunks of lc / 2 keys sorted in the LDS scratch (lc,
+// a power of two >= 128, elements of it), then merge-path rounds between p
+// and tmp (n elements).
+template <int NW>
+__device__ __forceinline__ void sort_hbm(u64 *p, i32 n, u64 *lds, i32 lc, u64 *tmp, const Blk<NW> &b) {
+	if (n <= 1) return;
+	const i32 cw = lc / 2;
+	for (i32 c0 = 0; c0 < n; c0 += cw) {
+		const i32 m = n - c0 < cw ? n - c0 : cw;
+		stage(lds, p + c0, m, b);
+		b.sync();
+		sort_runs(lds, m, lds + cw, 0, b);
+		for (i32 t = (i32)b.tid(); t < m; t += Blk<NW>::NT) p[c0 + t] = lds[cw + t];
+		b.sync();
+	}
+	u64 *src = p, *dst = tmp;
+	for (i32 wdt = cw; wdt < n; wdt <<= 1) {
+		merge_round(src, dst, n, wdt, b);
+		b.sync();
+		u64 *t = src;
+		src = dst;
+		dst = t;
+	}
+	if (src != p) {
+		for (i32 t = (i32)b.tid(); t < n; t += Blk<NW>::NT) p[t] = src[t];
 		b.sync();
 	}
 }
