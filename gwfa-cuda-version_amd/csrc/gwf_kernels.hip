@@ -1248,6 +1248,7 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 	// ---- P5: history H[hs ^ 1] = coalesce(H[hs] u runs of this step's exits)
 	const i32 cap_h = n_h + n_tt;  // bound on the merged interval count
 	if (p5_done) {
+		// (done beside the closure)
 	} else if (HBM && 2 * n_tt + 2 * n_h + 4 * cap_h <= w.sort_n) {
 		nh2 = hist_lds(w, b, Hcur_lo, Hcur_hi, n_h, n_tt, Hlo, Hhi);
 		hist_res = 2 * cap_h;
