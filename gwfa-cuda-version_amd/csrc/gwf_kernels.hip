@@ -238,8 +238,10 @@ struct WaveB {
 	static constexpr int NT = WAVE;
 	__device__ __forceinline__ u32 tid() const { return lane_id(); }
 	__device__ __forceinline__ u32 wid() const { return 0u; }
-	// LDS accesses of one wave are performed in order; this only keeps the
-	// compiler from moving memory accesses across the point
+	// A wave's memory accesses are performed in order (LDS, and global
+	// memory through the CU's vector cache, as the workgroup barrier already
+	// relies on for the waves of one CU); this only keeps the compiler from
+	// moving memory accesses across the point
 	__device__ __forceinline__ void sync() const {
 		__builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
 		__builtin_amdgcn_wave_barrier();
