@@ -168,7 +168,7 @@ struct SrvArgs {
 	const SrvEntry *ring;   // host-coherent, mask + 1 entries
 	SrvResult *res;         // host-coherent, mask + 1 results
 	const SrvCtl *ctl;      // host-coherent
-	u64 *claimed;           // device: requests claimed so far
+	u64 *claimed;           // device: next claim ticket (= requests claimed while no ticket is held)
 	const uint8_t *stage;   // host-coherent staging arena
 	uint8_t *pool;          // device pool
 	GwfBeat *beat;          // host-coherent, one per workgroup (may be null)

@@ -1545,7 +1545,7 @@ __device__ __forceinline__ u64 host_ld(const u64 *p) {
 }
 
 template <int NW>
-__global__ __launch_bounds__(WAVE *NW) void gwf_serve_kernel(SrvArgs a) {
+__global__ __launch_bounds__(WAVE *NW) __attribute__((amdgpu_waves_per_eu(4))) void gwf_serve_kernel(SrvArgs a) {
 	extern __shared__ u64 arena[];
 	__shared__ u64 vis[VIS_SLOTS];
 	__shared__ i32 red[2 * NW];
@@ -1562,34 +1562,48 @@ __global__ __launch_bounds__(WAVE *NW) void gwf_serve_kernel(SrvArgs a) {
 	const i32 sort_cap = arena_sort_cap(lds_elems(a.lc));
 	GwfBeat *beat = a.beat ? a.beat + blockIdx.x : nullptr;
 	i32 idle = 0;
-	u64 claimed_seen = 0;  // thread 0: a lower bound of the claim counter
 	for (;;) {
 		if (b.tid() == 0) {
-			// Idle polling reads only the host's tail word, with exponential
-			// back-off (1 to 32 sleeps of 127 x 64 cycles, ~4 to ~125 us): a
-			// thousand idle workgroups hammering one device counter or the
-			// host link would slow the busy ones down.  The claim counter is
-			// only touched when the tail passes the last count seen (c may be
-			// stale-low; a failed compare-exchange refreshes it).
+			// Claim by ticket: a fetch-add on the claim counter gives this
+			// workgroup the number of the next unclaimed request, with one
+			// atomic that always succeeds (compare-exchange claims by many idle
+			// workgroups mostly fail and queue up behind each other on the one
+			// address).  The workgroup then waits until the host has published
+			// its request, polling only the host's tail word: one sleep of
+			// 127 x 64 cycles (~3 us) between polls within 4 requests of the
+			// tail, else 1 to 32 growing with the time spent idle, 8 times as
+			// many from 64 requests ahead (measured: ~2000 idle workgroups
+			// polling host memory every ~100 us made a busy workgroup's read
+			// 2-5x slower).  On the stop bit it ends holding its ticket (the
+			// host puts the counter back to the tail).  After idle_polls empty
+			// polls it hands the ticket back, which only the holder of the last
+			// ticket issued can do, so tickets return newest first and, once
+			// every workgroup has ended by itself, the counter is again the
+			// first unclaimed request.
 			int64_t got = -1;
-			u64 c = claimed_seen;
+			u64 t = ~0ull;  // ticket held
 			for (;;) {
 				const u64 w = host_ld(&a.ctl->word);
 				const u64 tail = w & ~GWF_SRV_STOP;
-				if (c < tail) {
-					if (__hip_atomic_compare_exchange_strong(a.claimed, &c, c + 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-					                                         __HIP_MEMORY_SCOPE_AGENT)) {
-						got = (int64_t)c;
-						c += 1;
-						break;
-					}
-					continue;  // c now holds the current count
+				if (t == ~0ull) {
+					if (w & GWF_SRV_STOP) break;
+					t = __hip_atomic_fetch_add(a.claimed, (u64)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 				}
-				if ((w & GWF_SRV_STOP) || ++idle > a.idle_polls) break;
-				const i32 naps = idle < 40 ? 1 << (idle >> 3) : 32;
+				if (t < tail) {
+					got = (int64_t)t;
+					break;
+				}
+				if (w & GWF_SRV_STOP) break;
+				if (++idle > a.idle_polls) {
+					u64 last = t + 1;
+					if (__hip_atomic_compare_exchange_strong(a.claimed, &last, t, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+					                                         __HIP_MEMORY_SCOPE_AGENT))
+						break;
+				}
+				const u64 ahead = t - tail;
+				const i32 naps = ahead < 4 ? 1 : (idle < 40 ? 1 << (idle >> 3) : 32) << (ahead < 64 ? 0 : 3);
 				for (i32 z = 0; z < naps; ++z) __builtin_amdgcn_s_sleep(127);
 			}
-			claimed_seen = c;
 			if (got >= 0) {
 				s_claim = (int64_t)wall_clock64();
 				s_polls = idle;

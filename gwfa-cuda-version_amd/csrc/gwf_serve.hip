@@ -346,7 +346,8 @@ bool gwf_serve_s::device_check(int64_t now) {
 		a.mask = (u64)o.ring_slots - 1;
 		a.lc = cfg.lds;
 		a.mw = cfg.mw;
-		a.idle_polls = 40000;  // ~5 s of empty polls: the loop ends by itself if the host stops feeding it
+		// ~5 s of empty polls: the loop ends by itself if the host stops feeding it
+		a.idle_polls = gwf_env_int("GWF_SERVE_IDLE_POLLS", 40000);
 		for (i32 w = 0; w < cfg.wgs; ++w) beat_h[w].read = -1;
 		const hipError_t e = gwf_launch_serve(a, cfg, stream);
 		if (e != hipSuccess) {
@@ -365,8 +366,12 @@ bool gwf_serve_s::device_check(int64_t now) {
 void gwf_serve_s::park() {
 	if (!running) return;
 	__atomic_store_n(&ctl_h->word, tail | GWF_SRV_STOP, __ATOMIC_RELEASE);
-	const hipError_t e = hipStreamSynchronize(stream);
+	hipError_t e = hipStreamSynchronize(stream);
 	__atomic_store_n(&ctl_h->word, tail, __ATOMIC_RELEASE);
+	// every published request is done; the workgroups ended holding tickets
+	// past the tail, so the next loop's tickets start at the tail again
+	const u64 next = tail;
+	if (e == hipSuccess) e = hipMemcpy(claimed, &next, sizeof next, hipMemcpyHostToDevice);
 	running = false;
 	if (e != hipSuccess) {
 		char m[256];
@@ -505,10 +510,9 @@ int gwf_serve_start(gwf_dgraph_t *dg, const gwf_serve_opts_t *opts, gwf_serve_t 
 	if (o.state_budget_bytes <= 0) o.state_budget_bytes = (int64_t)1 << 30;
 	if (o.staging_bytes <= 0) o.staging_bytes = (int64_t)64 << 20;
 	if (o.ring_slots <= 0) o.ring_slots = 4096;
-	// measured on the uneven-length workload: 4 waves x 1024 states (4
-	// workgroups per CU) serves 20k requests in 443 ms, 2 x 256 (10 per CU,
-	// most of them idle under a 1 GiB budget) in 593-6600 ms; with a 16 GiB
-	// budget two-wave shapes stay 7-13x slower on 40k requests
+	// measured on the uneven-length workload (40k requests, 16 GiB budget):
+	// 4 waves x 1024 states (4 workgroups per CU) 367 ms, 2 x 256 434 ms,
+	// 2 x 128 451 ms
 	if (o.nw <= 0) o.nw = 4;
 	if (o.lds_states <= 0) o.lds_states = 1024;
 	if (o.park_ms <= 0) o.park_ms = 20;

@@ -130,3 +130,29 @@ def test_serve_parks_and_relaunches(gpu, c3):
         assert st["launches"] >= 2
     finally:
         dg.free()
+
+
+def test_serve_loop_ends_by_itself(gpu, c3, monkeypatch):
+    """Idle workgroups hand their claim tickets back newest first and end; the
+    next request relaunches the loop and is claimed at the right sequence."""
+    _say("case: serve loop ends after its idle polls (no park) and relaunches")
+    monkeypatch.setenv("GWF_SERVE_IDLE_POLLS", "300")
+    g, r, want = c3
+    dg = gpu.DeviceGraph(g)
+    try:
+        with gpu.Server(dg, park_ms=600000) as srv:
+            ids1 = srv.submit(r.subset(np.arange(40)))
+            res1 = srv.collect(40, timeout_s=60)
+            time.sleep(3.0)
+            ids2 = srv.submit(r.subset(np.arange(40, 60)))
+            res2 = srv.collect(20, timeout_s=60)
+            time.sleep(3.0)
+            ids3 = srv.submit(r.subset(np.arange(60, 61)))
+            res3 = srv.collect(1, timeout_s=60)
+            st = srv.stats()
+        _check_results(res1, want[:40], ids1)
+        _check_results(res2, want[40:60], ids2)
+        _check_results(res3, want[60:61], ids3)
+        assert st["launches"] == 3
+    finally:
+        dg.free()
