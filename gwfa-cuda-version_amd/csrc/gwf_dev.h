@@ -93,7 +93,7 @@ hipError_t gwf_launch_persistent(const DGraph &g, const DReads &r, const i32 *or
 // predicted wavefront size (~3.2 x predicted ED + 16; the predicted ED is
 // the read's length times the error rate its probe prefix showed).
 #define GWF_MAX_BINS 8
-#define GWF_COST_BUCKETS 64
+#define GWF_COST_BUCKETS 128  // GWF_MAX_BINS x this <= 1024 (one scan workgroup)
 struct BinBounds {
 	i32 n_bins;
 	i32 est_max[GWF_MAX_BINS];
