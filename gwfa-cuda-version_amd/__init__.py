@@ -181,6 +181,8 @@ def hip_lib():
         lib.gwf_serve_start.argtypes = [C.c_void_p, P(ServeOpts), P(C.c_void_p)]
         lib.gwf_serve_submit.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
                                          C.c_void_p, C.c_void_p, C.c_void_p]
+        lib.gwf_serve_submit_v1.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                            C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         lib.gwf_serve_poll.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_int32]
         lib.gwf_serve_drain.argtypes = [C.c_void_p, C.c_int32]
         lib.gwf_serve_stats.argtypes = [C.c_void_p, P(ServeStats)]
@@ -505,19 +507,27 @@ class Server:
     def clock_ns():
         return hip_lib().gwf_serve_clock_ns()
 
-    def submit(self, reads, release_ns=None):
+    def submit(self, reads, release_ns=None, v1=None):
         """Queue every read of `reads` (a Reads); release_ns (int64 array,
-        server clock) delays each one's visibility.  Returns the ids."""
+        server clock) delays each one's visibility; v1 (int, or int32 array:
+        end vertex per read, < 0 = anywhere; None = anywhere).  Returns the ids."""
         n = len(reads)
         ids = np.empty(max(n, 1), np.int64)
         rel = None
         if release_ns is not None:
             rel = np.ascontiguousarray(release_ns, dtype=np.int64)
             assert len(rel) == n
-        _check(self.lib.gwf_serve_submit(self.ptr, n, reads.seqs.ctypes.data, reads.off.ctypes.data,
-                                         reads.ql.ctypes.data, reads.v0.ctypes.data,
-                                         rel.ctypes.data if rel is not None else None, ids.ctypes.data),
-               "gwf_serve_submit")
+        if v1 is None:
+            _check(self.lib.gwf_serve_submit(self.ptr, n, reads.seqs.ctypes.data, reads.off.ctypes.data,
+                                             reads.ql.ctypes.data, reads.v0.ctypes.data,
+                                             rel.ctypes.data if rel is not None else None, ids.ctypes.data),
+                   "gwf_serve_submit")
+        else:
+            ends = np.ascontiguousarray(np.broadcast_to(np.asarray(v1, dtype=np.int32), (n,)))
+            _check(self.lib.gwf_serve_submit_v1(self.ptr, n, reads.seqs.ctypes.data, reads.off.ctypes.data,
+                                                reads.ql.ctypes.data, reads.v0.ctypes.data, ends.ctypes.data,
+                                                rel.ctypes.data if rel is not None else None, ids.ctypes.data),
+                   "gwf_serve_submit_v1")
         return ids[:n]
 
     def poll(self, max_n=4096, wait_ms=0):

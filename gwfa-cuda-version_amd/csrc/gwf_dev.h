@@ -145,7 +145,8 @@ struct SrvEntry {
 	i32 ql, v0;
 	Caps hc;       // HBM workspace capacities
 	u32 res_slot;
-	i32 pad[2];
+	i32 v1;        // end vertex (< 0: anywhere)
+	i32 pad;
 };
 struct SrvResult {
 	i32 ed, status;

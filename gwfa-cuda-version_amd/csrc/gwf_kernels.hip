@@ -1626,6 +1626,7 @@ __global__ __launch_bounds__(WAVE *NW) __attribute__((amdgpu_waves_per_eu(4))) v
 		e.ws_off = uni64(s_e.ws_off);
 		e.ql = __builtin_amdgcn_readfirstlane(s_e.ql);
 		e.v0 = __builtin_amdgcn_readfirstlane(s_e.v0);
+		e.v1 = __builtin_amdgcn_readfirstlane(s_e.v1);
 		e.hc.m = __builtin_amdgcn_readfirstlane(s_e.hc.m);
 		e.hc.x = __builtin_amdgcn_readfirstlane(s_e.hc.x);
 		e.hc.f = __builtin_amdgcn_readfirstlane(s_e.hc.f);
@@ -1646,7 +1647,7 @@ __global__ __launch_bounds__(WAVE *NW) __attribute__((amdgpu_waves_per_eu(4))) v
 		hb.sort_n = (i32)lds_elems(a.lc);
 		hb.mw = a.mw;
 		const u64 t0 = wall_clock64();
-		const ReadOut o = align_read<NW, true>(a.g, dq, e.ql, (u32)e.v0, -1, lb, hb, b, vis, beat, (i32)seq);
+		const ReadOut o = align_read<NW, true>(a.g, dq, e.ql, (u32)e.v0, e.v1, lb, hb, b, vis, beat, (i32)seq);
 		int64_t c = o.chars;
 #pragma unroll
 		for (int s = 32; s > 0; s >>= 1) c += __shfl_xor(c, s);

@@ -111,7 +111,7 @@ private:
 struct Req {
 	int64_t id = 0;
 	std::string bytes;
-	i32 ql = 0, v0 = 0;
+	i32 ql = 0, v0 = 0, v1 = -1;
 	int64_t t_arrival = 0;
 	int scale = 1;
 	i32 attempts = 0;
@@ -261,6 +261,7 @@ bool gwf_serve_s::admit_one(int64_t now) {
 	e.ws_off = ws_off;
 	e.ql = r.ql;
 	e.v0 = r.v0;
+	e.v1 = r.v1;
 	e.hc = hc;
 	e.res_slot = slot;
 	Flight &f = slots[slot];
@@ -573,6 +574,11 @@ int gwf_serve_start(gwf_dgraph_t *dg, const gwf_serve_opts_t *opts, gwf_serve_t 
 
 int gwf_serve_submit(gwf_serve_t *s, int32_t n, const char *seqs, const int64_t *off, const int32_t *ql,
                      const int32_t *v0, const int64_t *t_release_ns, int64_t *ids) {
+	return gwf_serve_submit_v1(s, n, seqs, off, ql, v0, nullptr, t_release_ns, ids);
+}
+
+int gwf_serve_submit_v1(gwf_serve_t *s, int32_t n, const char *seqs, const int64_t *off, const int32_t *ql,
+                        const int32_t *v0, const int32_t *v1, const int64_t *t_release_ns, int64_t *ids) {
 	if (!s || n < 0 || (n > 0 && (!seqs || !off || !ql))) {
 		gwf_set_error("gwf_serve_submit: bad argument");
 		return GWF_ERR_ARG;
@@ -590,10 +596,16 @@ int gwf_serve_submit(gwf_serve_t *s, int32_t n, const char *seqs, const int64_t 
 			gwf_set_error("gwf_serve_submit: request %d has v0 %d (n_vtx %u)", i, v, s->dg->n_vtx);
 			return GWF_ERR_ARG;
 		}
+		const i32 e = v1 && v1[i] >= 0 ? v1[i] : -1;
+		if (e >= 0 && (u32)e >= s->dg->n_vtx) {
+			gwf_set_error("gwf_serve_submit: request %d has v1 %d (n_vtx %u)", i, e, s->dg->n_vtx);
+			return GWF_ERR_ARG;
+		}
 		std::unique_ptr<Req> r(new Req());
 		r->bytes.assign(seqs + off[i], (size_t)ql[i]);
 		r->ql = ql[i];
 		r->v0 = v;
+		r->v1 = e;
 		r->t_arrival = t_release_ns ? std::max(now, t_release_ns[i]) : now;
 		batch.push_back(std::move(r));
 	}

@@ -297,6 +297,13 @@ int gwf_serve_start(gwf_dgraph_t *dg, const gwf_serve_opts_t *opts, gwf_serve_t 
 int gwf_serve_submit(gwf_serve_t *s, int32_t n, const char *seqs, const int64_t *off, const int32_t *ql,
                      const int32_t *v0, const int64_t *t_release_ns, int64_t *ids);
 
+/* gwf_serve_submit with an end vertex per request, as gwf_ed's v1 argument
+ * (gwfa.h:95 of the reference): v1[i] >= 0 ends request i at the last base
+ * of vertex v1[i] (gwf-ed.cu:1231-1254), v1[i] < 0 anywhere; v1 NULL: every
+ * request ends anywhere.  A request with no reachable end gets GWF_ED_NONE. */
+int gwf_serve_submit_v1(gwf_serve_t *s, int32_t n, const char *seqs, const int64_t *off, const int32_t *ql,
+                        const int32_t *v0, const int32_t *v1, const int64_t *t_release_ns, int64_t *ids);
+
 /* Move up to max finished results into out, waiting up to wait_ms for the
  * first one.  Returns the number moved (>= 0) or a GWF_ERR_* code. */
 int gwf_serve_poll(gwf_serve_t *s, gwf_serve_result_t *out, int32_t max, int32_t wait_ms);

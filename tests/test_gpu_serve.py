@@ -156,3 +156,30 @@ def test_serve_loop_ends_by_itself(gpu, c3, monkeypatch):
         assert st["launches"] == 3
     finally:
         dg.free()
+
+
+def test_serve_end_vertex(gpu, orc):
+    """Per-request end vertex (gwf_serve_submit_v1; v1 >= 0 ends at the last
+    base of v1, gwf-ed.cu:1231-1254): requests with different end vertices,
+    and ones that end anywhere, share one server, and each result equals
+    the oracle's for its own v1."""
+    _say("case: serve with per-request end vertices, mixed with v1 < 0")
+    g, r = gpu.make_workload(1, 40)
+    g.index()
+    ends = (5, g.n_vtx // 3, -1)
+    want = {e: orc.batch(g, r, v1=e)[0] for e in ends}
+    dg = gpu.DeviceGraph(g)
+    try:
+        # default shape, and one wave with the smallest LDS capacity (state spilled to HBM)
+        for opts in ({}, dict(nw=1, lds_states=64)):
+            with gpu.Server(dg, **opts) as srv:
+                ids = {e: srv.submit(r, v1=e) for e in ends}
+                res = srv.collect(len(ends) * len(r), timeout_s=120)
+            row = {int(i): k for k, i in enumerate(res["id"])}
+            for e in ends:
+                k = np.array([row[int(i)] for i in ids[e]])
+                assert (res["status"][k] == 0).all(), (opts, e, res["status"][k].tolist())
+                assert np.array_equal(res["ed"][k], want[e]), (opts, e, res["ed"][k].tolist(), want[e].tolist())
+                _say(f"  {opts} v1={e}: {int(res['spilled'][k].sum())} of {len(k)} spilled")
+    finally:
+        dg.free()
