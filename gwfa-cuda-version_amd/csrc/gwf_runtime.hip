@@ -355,9 +355,16 @@ int gwf_dgraph_upload(const gwf_graph_t *g, gwf_dgraph_t **out) {
 			set_err("gwf_dgraph_upload: arc %u (%u->%u, overlap %d) is invalid", i, u, w, o);
 			return GWF_ERR_ARG;
 		}
+		// The visited set keys a vertex entry by (target, query position), as
+		// gwf-ed.cu:1196 does.  With one overlap per target, every entry under
+		// a key is the same state; with several, the reference keeps the entry
+		// its FIFO closure reaches first, an order the level-parallel closure
+		// does not follow, so such graphs are refused rather than answered
+		// differently.
 		if (in_ol[w] >= 0 && in_ol[w] != o) {
-			set_err("gwf_dgraph_upload: vertex %u has in-arcs with different overlaps (%d, %d); "
-			        "the GPU path needs one overlap per target vertex", w, in_ol[w], o);
+			set_err("gwf_dgraph_upload: vertex %u has in-arcs with different overlaps (%d, %d); the "
+			        "reference's result then depends on its closure order, which the GPU does not follow",
+			        w, in_ol[w], o);
 			return GWF_ERR_ARG;
 		}
 		in_ol[w] = o;

@@ -223,3 +223,23 @@ def test_v1_skip_insertion_case(gw, orc):
     assert orc.dp(g, q, v1=13) == 1
     assert orc.ed(g, q, v1=13) == 8
     assert orc.ed(g, q[:-1], v1=13) == 0      # without the trailing base the end is exact
+
+
+def test_mixed_in_overlaps_make_the_result_order_dependent(gw, orc):
+    """Why the GPU path refuses a vertex whose in-arcs carry different
+    overlaps: the visited set keys a vertex entry by (target, query position)
+    (gwf-ed.cu:1196), so two arcs entering one vertex at one query position
+    with different overlaps give two different states under one key, and the
+    reference keeps whichever its FIFO closure reaches first.  Relabelling
+    the vertices (start kept at 0) then changes its answer; with one overlap
+    per target it never does (test_invariants)."""
+    seqs = ["CAA", "ACC", "GTGG"]
+    arcs = [(0, 1, 0), (1, 1, 2), (1, 2, 3), (1, 1, 1), (2, 0, 2), (2, 2, 1), (2, 1, 2)]
+    q = "GCGCTCGGAAGTATTATT"
+    g = gw.Graph.from_lists(seqs, arcs)
+    g.index()
+    perm = [0, 2, 1]
+    inv = {old: new for new, old in enumerate(perm)}
+    g2 = gw.Graph.from_lists([seqs[o] for o in perm], [(inv[a], inv[b], o) for a, b, o in arcs])
+    g2.index()
+    assert (orc.ed(g, q), orc.ed(g2, q), orc.dp(g, q)) == (12, 11, 11)
