@@ -319,12 +319,6 @@ __device__ __forceinline__ i32 upper_bound(const u64 *a, i32 n, u64 x) {
 	return lo;
 }
 
-__device__ __forceinline__ i32 next_pow2(i32 x) {
-	i32 n = 2;
-	while (n < x) n <<= 1;
-	return n;
-}
-
 // ------------------------------------------------------------ extension
 // LCE of graph bytes ts[k+1..] and query bytes q[d+k+1..] up to max_k: the
 // signed-max_k form of gwf_extend1 (gwf-ed.cu:806-839).  All lanes of the
