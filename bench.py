@@ -252,6 +252,12 @@ def main():
                                      "service = result time - start time of that read"},
         "roofline": roof,
         "gpu_counters": {"states": st["sum_states"], "chars": st["sum_chars"], "steps": st["sum_steps"]},
+        "spill": ({"reads_spilled": st["n_spill"], "read_fraction": round(st["n_spill"] / max(len(reads), 1), 4),
+                   "hbm_step_fraction": round(st["sum_hbm_steps"] / max(st["sum_steps"], 1), 4),
+                   "definition": "reads whose state outgrew the LDS capacity and moved to HBM; share of all "
+                                 "score steps run with the state in HBM"}
+                  if kernel == gw.KERNEL_PERSISTENT else
+                  {"definition": "plain kernel: every read's state is in HBM throughout"}),
     }
     if not args.no_cpu:
         cb, par, ost = cpu_baseline(gw, graph, reads, args.cpu_sample, eds)

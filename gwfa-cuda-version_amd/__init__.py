@@ -63,7 +63,7 @@ class SynthReadsOpts(C.Structure):
 class RunStats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("wall_ms", C.c_double), ("n_launch", C.c_int64),
                 ("n_retry", C.c_int64), ("sum_steps", C.c_int64), ("sum_states", C.c_int64),
-                ("sum_chars", C.c_int64), ("n_spill", C.c_int64)]
+                ("sum_chars", C.c_int64), ("n_spill", C.c_int64), ("sum_hbm_steps", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

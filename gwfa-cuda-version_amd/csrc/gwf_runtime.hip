@@ -853,6 +853,7 @@ int gwf_ed_run_timed(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t v1, uint32_t ma
 			acc.st.sum_states += c[1];
 			acc.st.sum_chars += c[2];
 			acc.st.n_spill += c[5] >= 0;
+			acc.st.sum_hbm_steps += c[5] >= 0 ? c[0] - c[5] : 0;
 			tmin = std::min(tmin, c[3]);
 		}
 		const double ns_per_tick = cx.wall_khz > 0 ? 1e6 / cx.wall_khz : 10.0;

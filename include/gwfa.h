@@ -157,6 +157,7 @@ typedef struct {
 	int64_t  sum_states;       /* sum over reads and steps of wavefront size */
 	int64_t  sum_chars;        /* characters compared by extension */
 	int64_t  n_spill;          /* reads whose state left LDS for HBM */
+	int64_t  sum_hbm_steps;    /* of sum_steps, those run with the state in HBM */
 } gwf_run_stats_t;
 
 /* Upload an indexed graph (gwf_ed_index() done) to the current device. */
