@@ -543,7 +543,10 @@ void heartbeat_wait(RunCtx &cx, const Heartbeat &hb) {
 	}
 }
 
-bool timed_launch(RunCtx &cx, RunAcc &acc, const std::function<hipError_t()> &launch, const Heartbeat *hb = nullptr) {
+// launches: alignment kernels the call starts (0 for a graph whose kernels
+// were counted when it was planned)
+bool timed_launch(RunCtx &cx, RunAcc &acc, const std::function<hipError_t()> &launch, const Heartbeat *hb = nullptr,
+                  int launches = 1) {
 	ACC(hipEventRecord(cx.e0, cx.s));
 	ACC(launch());
 	ACC(hipEventRecord(cx.e1, cx.s));
@@ -552,7 +555,7 @@ bool timed_launch(RunCtx &cx, RunAcc &acc, const std::function<hipError_t()> &la
 	float ms = 0;
 	ACC(hipEventElapsedTime(&ms, cx.e0, cx.e1));
 	acc.st.kernel_ms += ms;
-	acc.st.n_launch += 1;
+	acc.st.n_launch += launches;
 	return true;
 }
 
@@ -625,6 +628,9 @@ struct BinnedPlan {
 	i32 n = 0;
 	i32 v1 = -1;
 	bool concurrent = false;
+	// one bin and no more reads than resident workgroups: every read starts at
+	// launch whatever the order, so the probe and the binning are skipped
+	bool direct = false;
 	GwfBeat *beat = nullptr;  // heartbeat records: beat_stride per bin (probe: bin n_bins)
 	i32 beat_stride = 0;
 };
@@ -642,6 +648,9 @@ hipError_t enqueue_binned(RunCtx &cx, const gwf_dgraph_t *dg, const gwf_dreads_t
 	ENQ(gwf_launch_fill(cx.boff + 16, 1, 0, cx.s));
 	ENQ(gwf_launch_fill(cx.boff + 17, 1, p.n, cx.s));
 	ENQ(gwf_launch_fill(cx.head + 16, 1, 0, cx.s));
+	if (p.direct)  // reads in input order from the same one-range queue the probe uses
+		return gwf_launch_persistent(dg->g, rr, nullptr, cx.boff + 16, 0, cx.head + 16, p.bin[0], cx.scr[0],
+		                             gwf_ws_bytes(p.bin[0].hbm), cx.ed, cx.status, cx.cnt, p.beat, cx.s);
 	ENQ(gwf_launch_persistent(dg->g, rp, nullptr, cx.boff + 16, 0, cx.head + 16, p.probe, cx.scratch,
 	                          gwf_ws_bytes(p.probe.hbm), cx.pe, cx.ps, cx.cnt,
 	                          p.beat ? p.beat + (size_t)p.bb.n_bins * p.beat_stride : nullptr, cx.s));
@@ -686,6 +695,7 @@ bool run_binned(RunCtx &cx, RunAcc &acc, const gwf_dgraph_t *dg, const gwf_dread
 		ws_max = std::max(ws_max, w);
 	}
 	p.concurrent = ws_sum <= budget && env_int("GWF_BIN_STREAMS", 1) != 0;
+	p.direct = t.n_bins == 1 && n <= p.bin[0].wgs && env_int("GWF_DIRECT", 1) != 0;
 	ACC(grow(&cx.scratch, &cx.scratch_bytes, gwf_ws_bytes(p.probe.hbm) * (size_t)p.probe.wgs));
 	ACC(grow(&cx.pe, &cx.pe_c, 4 * (size_t)n));
 	ACC(grow(&cx.ps, &cx.ps_c, 4 * (size_t)n));
@@ -716,7 +726,7 @@ bool run_binned(RunCtx &cx, RunAcc &acc, const gwf_dgraph_t *dg, const gwf_dread
 		hb.what = "binned persistent";
 	}
 	const bool use_graph = graph_ok && env_int("GWF_GRAPH", 1) != 0 && !p.beat;
-	acc.st.n_launch += 1 + t.n_bins;
+	acc.st.n_launch += p.direct ? 1 : 1 + t.n_bins;
 	if (!use_graph) {
 		ACC(hipEventRecord(cx.e0, cx.s));
 		ACC(enqueue_binned(cx, dg, dr, p));
@@ -730,7 +740,7 @@ bool run_binned(RunCtx &cx, RunAcc &acc, const gwf_dgraph_t *dg, const gwf_dread
 	}
 	// everything the captured launches point at or were sized by
 	std::vector<uintptr_t> key = {(uintptr_t)dg, (uintptr_t)dr, (uintptr_t)n, (uintptr_t)tv, (uintptr_t)maxql, (uintptr_t)(v1 + 1),
-	                              (uintptr_t)p.concurrent, (uintptr_t)cx.ed, (uintptr_t)cx.status, (uintptr_t)cx.cnt,
+	                              (uintptr_t)p.concurrent, (uintptr_t)p.direct, (uintptr_t)cx.ed, (uintptr_t)cx.status, (uintptr_t)cx.cnt,
 	                              (uintptr_t)cx.scratch, (uintptr_t)cx.pe, (uintptr_t)cx.ps, (uintptr_t)cx.key,
 	                              (uintptr_t)cx.ord};
 	for (int b = 0; b < GWF_MAX_BINS; ++b) key.push_back((uintptr_t)cx.scr[b]);
@@ -754,7 +764,7 @@ bool run_binned(RunCtx &cx, RunAcc &acc, const gwf_dgraph_t *dg, const gwf_dread
 		ACC(e3);
 		cx.gkey = key;
 	}
-	return timed_launch(cx, acc, [&] { return hipGraphLaunch(cx.gexec, cx.s); });
+	return timed_launch(cx, acc, [&] { return hipGraphLaunch(cx.gexec, cx.s); }, nullptr, 0);
 }
 
 #undef ACC

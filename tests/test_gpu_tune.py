@@ -109,3 +109,25 @@ def test_heartbeat_progress_lines(gpu, capfd, monkeypatch):
     assert len(lines) >= 2, err[-2000:]
     assert "busy=" in lines[0] and "max_step=" in lines[0]
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("direct", ["1", "0"])
+def test_single_bin_direct_or_probed(gpu, mix, monkeypatch, direct):
+    """One bin and no more reads than resident workgroups: the run launches
+    the bin kernel on the reads in input order (GWF_DIRECT=1: one launch) or
+    probes and bins them first (GWF_DIRECT=0: two launches, the probe and the
+    bin); both give the oracle's EDs, eagerly and from the captured graph."""
+    _say(f"case: single-bin run of {len(mix[1])} reads, GWF_DIRECT={direct}")
+    monkeypatch.setenv("GWF_DIRECT", direct)
+    g, r, want = mix
+    dg, dr = gpu.DeviceGraph(g), gpu.DeviceReads(r)
+    try:
+        gpu.tune_set([dict(est_max=1 << 30, nw=2, lds_states=128, merge_runs=0)])
+        for rep in range(3):  # capture, then replays
+            got, st = gpu.ed_run(dg, dr, kernel=gpu.KERNEL_PERSISTENT)
+            assert np.array_equal(got, want), rep
+            assert st["n_launch"] == (1 if direct == "1" else 2), st["n_launch"]
+    finally:
+        gpu.tune_set(None)
+        dr.free()
+        dg.free()
