@@ -180,6 +180,8 @@ def test_lds_spill_to_hbm(gpu, orc, monkeypatch):
     got, st = _gpu_eds(gpu, g, r, 1)
     assert st["n_spill"] > 0 and st["n_retry"] == 0
     assert np.array_equal(got, want)
+    # steps run in HBM: some, fewer than all (each spilled read starts in LDS)
+    assert 0 < st["sum_hbm_steps"] < st["sum_steps"]
 
 
 @pytest.mark.parametrize("nw,margin", [(1, 1 << 20), (2, 1 << 20), (2, 0)])
