@@ -3,10 +3,15 @@
 on one MI355X, with every ED checked bit-exact on the CPU-baseline sample.
 
 A step is one pass of the alignment over the whole read set of the workload
-(default: 20,000 x 5 kb reads, errors 1/5/10 % in equal thirds, against a
-1 Mb-backbone bubble graph, every read starting at offset 0 of a sampled
-vertex).  Graph and reads are resident in HBM before the timed region; the
-timed region is K calls of gwf_ed_run (launches + the 4 B/read ED copy-back).
+(default: the uneven batch, 100,000 reads with log-normal lengths of median
+3 kb clipped to 500 b-20 kb and error rates uniform on 1-10 %, against a
+5 Mb-backbone graph with nested bubbles and indel alleles up to 50 bp, every
+read starting at offset 0 of a sampled vertex).  Graph and reads are resident
+in HBM before the timed region; the timed region is K calls of gwf_ed_run
+(launches + the 4 B/read ED copy-back).  Outside it the same reads are aligned
+once by the plain one-wavefront-per-read launch (the persistent/plain speed-up)
+and every read by the CPU oracle on the host cores (the CPU baseline, and the
+parity check of every ED).
 
 --serve measures the serving loop instead (the uneven-length workload's
 graph resident, Poisson arrivals at swept fractions of the saturated request
@@ -44,18 +49,47 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(gw, graph, reads, sample, gpu_eds):
-    """Oracle (CPU port of gwf_ed) on the first `sample` reads, all host
-    threads up to 16; also the parity check of those reads."""
+def host_cores():
+    """Cores this process may use: its CPU affinity, limited to the thread
+    count the host allots to it (OMP_NUM_THREADS, when set)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    lim = os.environ.get("OMP_NUM_THREADS", "")
+    if lim.isdigit() and int(lim) > 0:
+        n = min(n, int(lim))
+    return max(1, n)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(gw, graph, reads, sample, sample_1t, gpu_eds):
+    """Oracle (CPU port of gwf_ed): the first `sample` reads on every host core
+    this process may use (also the parity check of those reads' EDs), then the
+    first `sample_1t` of them on one thread."""
     sys.path.insert(0, ROOT)
     import oracle
     idx = np.arange(min(sample, len(reads)))
     sub = reads.subset(idx)
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = host_cores()
     t0 = time.perf_counter()
     eds, st, used = oracle.batch(graph, sub, threads=threads)
     dt = time.perf_counter() - t0
     mism = int((eds != gpu_eds[idx]).sum())
+    sub1 = reads.subset(np.arange(min(sample_1t, len(idx))))
+    t1 = time.perf_counter()
+    eds1, _, _ = oracle.batch(graph, sub1, threads=1)
+    dt1 = time.perf_counter() - t1
+    mism += int((eds1 != gpu_eds[:len(sub1)]).sum())
+    log(f"cpu baseline: {sub.bases() / dt / 1e6:.2f} M bases/s on {used} threads ({dt:.1f} s), "
+        f"{sub1.bases() / dt1 / 1e6:.3f} M bases/s on 1 thread ({dt1:.1f} s)")
     return {
         "value": sub.bases() / dt,
         "unit": "bases/s",
@@ -63,6 +97,11 @@ def cpu_baseline(gw, graph, reads, sample, gpu_eds):
         "kind": "port",
         "sample": f"first {len(sub)} reads of the workload ({sub.bases()} bases), "
                   f"{dt:.2f} s wall on {used} threads",
+        "value_1t": sub1.bases() / dt1,
+        "sample_1t": f"first {len(sub1)} reads ({sub1.bases()} bases), {dt1:.2f} s wall on 1 thread",
+        "model": cpu_model(),
+        "host_cpus": os.cpu_count(),
+        "threads_rule": "every core of this process's CPU affinity, limited to OMP_NUM_THREADS when set",
     }, {"reads": len(sub), "mismatches": mism}, st
 
 
@@ -141,12 +180,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=2, help="workload 1-4 (default 2)")
+    ap.add_argument("--config", type=int, default=3, help="workload 1-4 (default 3, the uneven batch)")
     ap.add_argument("--kernel", choices=["plain", "persistent"], default="persistent")
     ap.add_argument("--reads", type=int, default=None, help="use only the first N reads")
-    ap.add_argument("--cpu-sample", type=int, default=20000,
-                    help="reads timed on the CPU oracle and parity-checked (default: the whole config-2 workload)")
+    ap.add_argument("--cpu-sample", type=int, default=None,
+                    help="reads timed on the CPU oracle on all host cores and parity-checked (default: every read)")
+    ap.add_argument("--cpu-sample-1t", type=int, default=1500,
+                    help="of those, reads timed again on one thread (default 1500)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="persistent kernel: skip the plain-launch comparison run")
     ap.add_argument("--tune", default=None, help="persistent kernel: load this bin table (no autotuning)")
     ap.add_argument("--save-tune", default=None, help="persistent kernel: write the table used to this file")
     ap.add_argument("--autotune", type=int, default=-1,
@@ -259,8 +302,25 @@ def main():
                   if kernel == gw.KERNEL_PERSISTENT else
                   {"definition": "plain kernel: every read's state is in HBM throughout"}),
     }
+    if kernel == gw.KERNEL_PERSISTENT and not args.no_plain:
+        # the plain one-wavefront-per-read launch on the same resident reads,
+        # outside the timed region: one warm-up run, one measured run
+        gw.ed_run(dg, dr, kernel=gw.KERNEL_PLAIN)
+        tp = time.perf_counter()
+        peds, pst = gw.ed_run(dg, dr, kernel=gw.KERNEL_PLAIN)
+        pdt = time.perf_counter() - tp
+        if not np.array_equal(peds, eds):
+            raise SystemExit("bench.py: plain and persistent EDs differ")
+        out["vs_plain"] = {"plain_bases_per_s": round(bases / pdt, 1), "plain_ms": round(pdt * 1e3, 3),
+                           "plain_kernel_ms": round(pst["kernel_ms"], 3), "plain_launches": pst["n_launch"],
+                           "speedup": round(value / (bases / pdt), 3),
+                           "definition": "persistent bases/s over the plain launch's bases/s (wall time of one "
+                                         "gwf_ed_run each, same resident reads); plain = one 64-lane wavefront per "
+                                         "read, batches of the budget spread over 16 streams"}
+        log(f"plain: {pdt * 1e3:.1f} ms ({pst['n_launch']} launches); persistent/plain {out['vs_plain']['speedup']}x")
     if not args.no_cpu:
-        cb, par, ost = cpu_baseline(gw, graph, reads, args.cpu_sample, eds)
+        sample = len(reads) if args.cpu_sample is None else args.cpu_sample
+        cb, par, ost = cpu_baseline(gw, graph, reads, sample, args.cpu_sample_1t, eds)
         out["cpu_baseline"] = cb
         out["parity_sample"] = par
         n = par["reads"]

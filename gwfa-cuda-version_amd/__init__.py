@@ -165,6 +165,10 @@ def hip_lib():
                                           P(C.c_void_p)]
         lib.gwf_dreads_free.argtypes = [C.c_void_p]
         lib.gwf_dreads_free.restype = None
+        lib.gwf_dgraph_generation.argtypes = [C.c_void_p]
+        lib.gwf_dgraph_generation.restype = C.c_uint64
+        lib.gwf_dreads_generation.argtypes = [C.c_void_p]
+        lib.gwf_dreads_generation.restype = C.c_uint64
         lib.gwf_ed_run.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_uint32, C.c_int32,
                                    C.c_void_p, P(RunStats)]
         lib.gwf_ed_run_timed.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_uint32, C.c_int32,
@@ -441,6 +445,11 @@ class DeviceGraph:
             pass
 
 
+    @property
+    def generation(self):
+        return int(self.lib.gwf_dgraph_generation(self.ptr))
+
+
 class DeviceReads:
     def __init__(self, reads):
         self.lib = hip_lib()
@@ -449,6 +458,10 @@ class DeviceReads:
         _check(self.lib.gwf_dreads_upload(self.n, reads.seqs.ctypes.data, reads.off.ctypes.data,
                                           reads.ql.ctypes.data, reads.v0.ctypes.data, C.byref(self.ptr)),
                "gwf_dreads_upload")
+
+    @property
+    def generation(self):
+        return int(self.lib.gwf_dreads_generation(self.ptr))
 
     def free(self):
         if self.ptr:
@@ -460,6 +473,21 @@ class DeviceReads:
             self.free()
         except Exception:
             pass
+
+
+def ed_batch(dgraph, reads, kernel=KERNEL_PLAIN, v1=-1):
+    """gwf_ed_batch: upload `reads` (a Reads) from host pointers, align, free;
+    returns (EDs, stats)."""
+    n = len(reads)
+    ptrs = (C.c_char_p * max(n, 1))()
+    base = reads.seqs.ctypes.data
+    for i in range(n):
+        ptrs[i] = C.cast(base + int(reads.off[i]), C.c_char_p)
+    out = np.empty(max(n, 1), dtype=np.int32)
+    st = RunStats()
+    _check(hip_lib().gwf_ed_batch(dgraph.ptr, n, ptrs, reads.ql.ctypes.data, reads.v0.ctypes.data, v1, 0, kernel,
+                                  out.ctypes.data, C.byref(st)), "gwf_ed_batch")
+    return out[:n], st.as_dict()
 
 
 def ed_run(dgraph, dreads, kernel=KERNEL_PLAIN, v1=-1, timed=False):

@@ -74,9 +74,17 @@ struct PersistCfg {
 size_t gwf_ws_bytes(const Caps &c);
 size_t gwf_lds_bytes(const Caps &c);
 
-hipError_t gwf_launch_plain(const DGraph &g, const DReads &r, const i32 *read_idx, i32 n, const Caps &caps,
-                            u64 *scratch, size_t ws_bytes, i32 *out_ed, i32 *out_status, int64_t *out_counts,
+// Plain kernel: one workgroup (one wave) per read_idx[0..n).  Read j's
+// workspace is scratch + ws_off[j] (u64 elements) with the capacities of
+// ws_a[j] (Caps {2a, a, a, a, a}): every read gets a workspace of its own size.
+hipError_t gwf_launch_plain(const DGraph &g, const DReads &r, const i32 *read_idx, i32 n, const u64 *ws_off,
+                            const i32 *ws_a, u64 *scratch, i32 *out_ed, i32 *out_status, int64_t *out_counts,
                             hipStream_t st);
+// The same kernel with equal workspaces: block j aligns read read_idx[j]
+// (read_idx == nullptr: read base + j) in scratch + j * gwf_ws_bytes(c) / 8,
+// c = Caps {2a, a, a, a, a}.  out_counts may be null.
+hipError_t gwf_launch_plain_uniform(const DGraph &g, const DReads &r, const i32 *read_idx, i32 base, i32 n, i32 a,
+                                    u64 *scratch, i32 *out_ed, i32 *out_status, int64_t *out_counts, hipStream_t st);
 
 // Persistent kernel: cfg.wgs workgroups pull the reads of bin `bin`,
 // order[bin_off[bin] .. bin_off[bin + 1]) (device arrays; order == nullptr:

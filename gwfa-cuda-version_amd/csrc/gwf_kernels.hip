@@ -806,6 +806,8 @@ __host__ __device__ inline size_t ws_elems(const Caps &c) {
 	return 2 * (size_t)c.m + c.x + 2 * (size_t)c.f + c.t + 4 * (size_t)c.h + (size_t)(c.x > c.t ? c.x : c.t) + 8;
 }
 size_t gwf_ws_bytes(const Caps &c) { return ((ws_elems(c) * 8 + 255) / 256) * 256; }
+// elements between equal plain-kernel workspaces (gwf_ws_bytes / 8)
+__host__ __device__ inline u64 plain_stride(const Caps &c) { return (ws_elems(c) * 8 + 255) / 256 * 32; }
 __host__ __device__ inline size_t lds_elems(const Caps &c) {
 	return 2 * (size_t)c.m + c.x + 2 * (size_t)c.f + c.t + 4 * (size_t)c.h;
 }
@@ -1428,23 +1430,29 @@ __device__ __forceinline__ void write_out(const ReadOut &o, i32 r, u64 t0, i32 *
 		b.sync();
 	}
 	if (b.tid() == 0) {
-		int64_t *cc = out_counts + GWF_NCOUNT * (size_t)r;
 		out_ed[r] = o.ed;
 		out_status[r] = o.status;
-		cc[0] = o.steps;
-		cc[1] = o.states;
-		cc[2] = c;
-		cc[3] = (int64_t)t0;
-		cc[4] = (int64_t)wall_clock64();
-		cc[5] = o.spill_step;
+		if (out_counts) {
+			int64_t *cc = out_counts + GWF_NCOUNT * (size_t)r;
+			cc[0] = o.steps;
+			cc[1] = o.states;
+			cc[2] = c;
+			cc[3] = (int64_t)t0;
+			cc[4] = (int64_t)wall_clock64();
+			cc[5] = o.spill_step;
+		}
 	}
 }
 
 // ------------------------------------------------------------ kernels
-// Plain: one wavefront per read, grid = reads, every array in HBM.
-__global__ __launch_bounds__(WAVE) void gwf_plain_kernel(DGraph g, DReads rd, const i32 *read_idx, i32 n, Caps caps,
-                                                         u64 *scratch, u64 ws_elems_per_read, i32 *out_ed,
-                                                         i32 *out_status, int64_t *out_counts) {
+// Plain: one wavefront per read, grid = reads, every array in HBM.  Block j
+// aligns read read_idx[j] (read_idx == nullptr: read base + j) in a workspace
+// of its own size (ws_off[j], ws_a[j]), or, with ws_off == nullptr, in slot j
+// of equal workspaces of capacity a_uniform.
+__global__ __launch_bounds__(WAVE) void gwf_plain_kernel(DGraph g, DReads rd, const i32 *read_idx, i32 base, i32 n,
+                                                         const u64 *ws_off, const i32 *ws_a, i32 a_uniform,
+                                                         u64 *scratch, i32 *out_ed, i32 *out_status,
+                                                         int64_t *out_counts) {
 	__shared__ u64 lds_sort[SORT_CHUNK];
 	__shared__ u64 vis[VIS_SLOTS];
 	__shared__ i32 red[2];
@@ -1452,9 +1460,13 @@ __global__ __launch_bounds__(WAVE) void gwf_plain_kernel(DGraph g, DReads rd, co
 	__shared__ i32 xch[4];
 	const i32 blk = blockIdx.x;
 	if (blk >= n) return;
-	const i32 r = read_idx ? read_idx[blk] : blk;
+	const i32 r = read_idx ? read_idx[blk] : base + blk;
 	Blk<1> b{red, redu, xch};
-	Bufs w = carve(scratch + (size_t)blk * ws_elems_per_read, caps, true);
+	const i32 a = ws_off ? ws_a[blk] : a_uniform;
+	Caps caps;
+	caps.m = 2 * a;
+	caps.x = caps.f = caps.t = caps.h = a;
+	Bufs w = carve(scratch + (ws_off ? ws_off[blk] : (u64)blk * plain_stride(caps)), caps, true);
 	w.sort_lds = lds_sort;
 	w.sort_cap = SORT_CHUNK;
 	w.sort_n = SORT_CHUNK;
@@ -1599,6 +1611,10 @@ __global__ __launch_bounds__(WAVE *NW) __attribute__((amdgpu_waves_per_eu(4))) v
 				for (i32 z = 0; z < naps; ++z) __builtin_amdgcn_s_sleep(127);
 			}
 			if (got >= 0) {
+				// pairs with the host's release store of the tail: the ring
+				// entry and the staged read bytes it published before that
+				// store are read after this point
+				__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
 				s_claim = (int64_t)wall_clock64();
 				s_polls = idle;
 				const u64 *src = (const u64 *)&a.ring[(u64)got & a.mask];
@@ -1772,12 +1788,20 @@ hipError_t gwf_launch_binning(const i32 *probe_ed, const i32 *probe_status, cons
 	return hipGetLastError();
 }
 
-hipError_t gwf_launch_plain(const DGraph &g, const DReads &r, const i32 *read_idx, i32 n, const Caps &caps,
-                            u64 *scratch, size_t ws_bytes, i32 *out_ed, i32 *out_status, int64_t *out_counts,
+hipError_t gwf_launch_plain(const DGraph &g, const DReads &r, const i32 *read_idx, i32 n, const u64 *ws_off,
+                            const i32 *ws_a, u64 *scratch, i32 *out_ed, i32 *out_status, int64_t *out_counts,
                             hipStream_t st) {
 	if (n <= 0) return hipSuccess;
-	hipLaunchKernelGGL(gwf_plain_kernel, dim3(n), dim3(WAVE), 0, st, g, r, read_idx, n, caps, scratch,
-	                   (u64)(ws_bytes / 8), out_ed, out_status, out_counts);
+	hipLaunchKernelGGL(gwf_plain_kernel, dim3(n), dim3(WAVE), 0, st, g, r, read_idx, 0, n, ws_off, ws_a, 0, scratch,
+	                   out_ed, out_status, out_counts);
+	return hipGetLastError();
+}
+
+hipError_t gwf_launch_plain_uniform(const DGraph &g, const DReads &r, const i32 *read_idx, i32 base, i32 n, i32 a,
+                                    u64 *scratch, i32 *out_ed, i32 *out_status, int64_t *out_counts, hipStream_t st) {
+	if (n <= 0) return hipSuccess;
+	hipLaunchKernelGGL(gwf_plain_kernel, dim3(n), dim3(WAVE), 0, st, g, r, read_idx, base, n, nullptr, nullptr, a,
+	                   scratch, out_ed, out_status, out_counts);
 	return hipGetLastError();
 }
 

@@ -1,6 +1,8 @@
 // gwf_policy.h — host-side sizing policy shared by the batch runtime and the
 // server: workspace capacities per read length, LDS capacities, error text.
 #pragma once
+#include <vector>
+
 #include "gwf_dev.h"
 #include "gwfa.h"
 
@@ -16,9 +18,25 @@ Caps gwf_lds_caps(i32 m);
 // Set this thread's gwf_last_error() text.
 void gwf_set_error(const char *fmt, ...);
 
+// A number unique to every upload in the process (never 0, never reused), so
+// that state derived from a handle (a captured graph's device pointers) can
+// tell a handle from a later one that the allocator placed at the same host
+// address.
+uint64_t gwf_next_generation();
+
 // Device graph handle (gwf_dgraph_upload).
 struct gwf_dgraph_s {
 	DGraph g;
 	void *mem = nullptr;
 	u32 n_vtx = 0;
+	uint64_t gen = 0;
+};
+
+// Device read-set handle (gwf_dreads_upload); ql and v0 are host copies.
+struct gwf_dreads_s {
+	DReads r;
+	const i32 *ql_probe = nullptr;  // device: min(ql, probe length) per read
+	void *mem = nullptr;
+	uint64_t gen = 0;
+	std::vector<i32> ql, v0;
 };

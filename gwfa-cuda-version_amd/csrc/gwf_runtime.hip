@@ -2,6 +2,7 @@
 // index, device-resident graph and reads, batched alignment with automatic
 // re-runs of reads that outgrow their workspace, and the host-callable gwf_ed.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <functional>
 #include <cstdarg>
@@ -41,6 +42,7 @@ void set_err(const char *fmt, ...) {
 	} while (0)
 
 constexpr i32 kProbeLen = 256;    // prefix aligned to estimate each read's error rate
+constexpr int kPlainStreams = 16; // streams of the plain launches (as the reference's driver, main.cu:22)
 
 int device_ready() {
 	int n = 0;
@@ -53,12 +55,10 @@ int device_ready() {
 
 }  // namespace
 
-struct gwf_dreads_s {
-	DReads r;
-	const i32 *ql_probe = nullptr;  // device: min(ql, kProbeLen) per read
-	void *mem = nullptr;
-	std::vector<i32> ql, v0;
-};
+uint64_t gwf_next_generation() {
+	static std::atomic<uint64_t> next{1};
+	return next.fetch_add(1, std::memory_order_relaxed);
+}
 
 namespace {
 
@@ -74,6 +74,8 @@ struct RunCtx {
 	hipEvent_t e0 = nullptr, e1 = nullptr;
 	i32 *ed = nullptr, *status = nullptr, *idx = nullptr, *head = nullptr;
 	int64_t *cnt = nullptr;
+	u64 *woff = nullptr;  // plain launches: per-read workspace offsets (elements) and sizes
+	i32 *wa = nullptr;
 	size_t n_cap = 0;
 	u64 *scratch = nullptr;
 	size_t scratch_bytes = 0;
@@ -89,6 +91,11 @@ struct RunCtx {
 	// graph of the last binned run, replayed while nothing it captured changes
 	hipGraphExec_t gexec = nullptr;
 	std::vector<uintptr_t> gkey;
+	// plain launches: one workspace arena per stream
+	hipStream_t pst[kPlainStreams] = {};
+	hipEvent_t ev_pj[kPlainStreams] = {};
+	u64 *pscr[kPlainStreams] = {};
+	size_t pscr_c[kPlainStreams] = {};
 };
 RunCtx g_run;
 
@@ -107,8 +114,9 @@ int ctx_prepare(RunCtx &c, size_t n) {
 	int dev = 0;
 	HIPCHK(hipGetDevice(&dev));
 	if (c.dev != dev) {  // first use (device buffers of another device are dropped)
-		c.ed = c.status = c.idx = c.head = nullptr;
+		c.ed = c.status = c.idx = c.head = c.wa = nullptr;
 		c.cnt = nullptr;
+		c.woff = nullptr;
 		c.scratch = nullptr;
 		c.n_cap = c.scratch_bytes = 0;
 		HIPCHK(hipStreamCreate(&c.s));
@@ -124,6 +132,11 @@ int ctx_prepare(RunCtx &c, size_t n) {
 			HIPCHK(hipEventCreateWithFlags(&c.ev_join[b], hipEventDisableTiming));
 		}
 		HIPCHK(hipEventCreateWithFlags(&c.ev_fork, hipEventDisableTiming));
+		for (int k = 0; k < kPlainStreams; ++k) {
+			HIPCHK(hipStreamCreate(&c.pst[k]));
+			HIPCHK(hipEventCreateWithFlags(&c.ev_pj[k], hipEventDisableTiming));
+			c.pscr[k] = nullptr, c.pscr_c[k] = 0;
+		}
 		HIPCHK(gwf_kernels_init());
 		c.pe = c.ps = c.key = c.ord = nullptr;
 		c.pe_c = c.ps_c = c.key_c = c.ord_c = 0;
@@ -135,14 +148,25 @@ int ctx_prepare(RunCtx &c, size_t n) {
 	if (c.n_cap < n) {
 		const size_t m = std::max<size_t>(n, 1024);
 		size_t c0 = c.n_cap * 4, c1 = c.n_cap * 4, c2 = c.n_cap * 4, c3 = c.n_cap * 8 * GWF_NCOUNT;
+		size_t c4 = c.n_cap * 8, c5 = c.n_cap * 4;
 		c.n_cap = 0;
 		HIPCHK(grow(&c.ed, &c0, 4 * m));
 		HIPCHK(grow(&c.status, &c1, 4 * m));
 		HIPCHK(grow(&c.idx, &c2, 4 * m));
 		HIPCHK(grow(&c.cnt, &c3, 8 * GWF_NCOUNT * m));
+		HIPCHK(grow(&c.woff, &c4, 8 * m));
+		HIPCHK(grow(&c.wa, &c5, 4 * m));
 		c.n_cap = m;
 	}
 	return GWF_OK;
+}
+
+// device bytes of the workspaces the context holds (reusable by the next run)
+size_t ctx_held(const RunCtx &c) {
+	size_t h = c.scratch_bytes;
+	for (int b = 0; b < GWF_MAX_BINS; ++b) h += c.scr_c[b];
+	for (int k = 0; k < kPlainStreams; ++k) h += c.pscr_c[k];
+	return h;
 }
 
 int64_t env_int(const char *name, int64_t dflt) {
@@ -271,8 +295,31 @@ gwf_tune_t tune_current(u64 *version) {
 	return g_tune;
 }
 
+// gwf_ed's device copies, one per graph pointer.  An entry is dropped by
+// gwf_ed_index and gwf_cleanup on that pointer (re-indexing rebuilds arc and
+// aux, so the upload no longer describes the graph), and is also replaced
+// when the graph's counts or array pointers no longer match the ones it was
+// uploaded from.
+struct CachedGraph {
+	gwf_dgraph_t *dg = nullptr;
+	u32 n_vtx = 0, n_arc = 0;
+	const void *len = nullptr, *seq = nullptr, *arc = nullptr, *aux = nullptr;
+	bool matches(const gwf_graph_t *g) const {
+		return n_vtx == g->n_vtx && n_arc == g->n_arc && len == g->len && seq == g->seq && arc == g->arc &&
+		       aux == g->aux;
+	}
+};
 std::mutex g_cache_mu;
-std::unordered_map<const gwf_graph_t *, gwf_dgraph_t *> g_cache;
+std::unordered_map<const gwf_graph_t *, CachedGraph> g_cache;
+
+void cache_drop(const gwf_graph_t *g) {
+	std::lock_guard<std::mutex> lk(g_cache_mu);
+	auto it = g_cache.find(g);
+	if (it != g_cache.end()) {
+		gwf_dgraph_free(it->second.dg);
+		g_cache.erase(it);
+	}
+}
 
 }  // namespace
 
@@ -295,6 +342,7 @@ const char *gwf_last_error(void) { return g_err.c_str(); }
 // main.cu:246-270: sort arcs by a = src << 32 | dst, aux[v] = first << 32 | count.
 void gwf_ed_index(gwf_graph_t *g) {
 	if (!g) return;
+	cache_drop(g);
 	free(g->aux);
 	g->aux = (uint64_t *)calloc(g->n_vtx ? g->n_vtx : 1, sizeof(uint64_t));
 	if (!g->aux || g->n_arc == 0) return;
@@ -311,14 +359,7 @@ void gwf_ed_index(gwf_graph_t *g) {
 
 void gwf_cleanup(gwf_graph_t *g) {
 	if (!g) return;
-	{
-		std::lock_guard<std::mutex> lk(g_cache_mu);
-		auto it = g_cache.find(g);
-		if (it != g_cache.end()) {
-			gwf_dgraph_free(it->second);
-			g_cache.erase(it);
-		}
-	}
+	cache_drop(g);
 	free(g->aux);
 	g->aux = nullptr;
 }
@@ -403,6 +444,7 @@ int gwf_dgraph_upload(const gwf_graph_t *g, gwf_dgraph_t **out) {
 	dg->g.n_vtx = nv;
 	dg->g.total_len = (u32)total;
 	dg->n_vtx = nv;
+	dg->gen = gwf_next_generation();
 	if (e != hipSuccess) {
 		set_err("gwf_dgraph_upload: hipMemcpy failed: %s", hipGetErrorString(e));
 		gwf_dgraph_free(dg);
@@ -471,6 +513,7 @@ int gwf_dreads_upload(int32_t n, const char *seqs, const int64_t *off, const int
 	dr->ql_probe = (const i32 *)p;
 	dr->r.n = n;
 	dr->r.v1 = -1;
+	dr->gen = gwf_next_generation();
 	if (e != hipSuccess) {
 		set_err("gwf_dreads_upload: hipMemcpy failed: %s", hipGetErrorString(e));
 		gwf_dreads_free(dr);
@@ -479,6 +522,9 @@ int gwf_dreads_upload(int32_t n, const char *seqs, const int64_t *off, const int
 	*out = dr;
 	return GWF_OK;
 }
+
+uint64_t gwf_dgraph_generation(const gwf_dgraph_t *dg) { return dg ? dg->gen : 0; }
+uint64_t gwf_dreads_generation(const gwf_dreads_t *dr) { return dr ? dr->gen : 0; }
 
 void gwf_dreads_free(gwf_dreads_t *dr) {
 	if (!dr) return;
@@ -559,30 +605,68 @@ bool timed_launch(RunCtx &cx, RunAcc &acc, const std::function<hipError_t()> &la
 	return true;
 }
 
-// Plain kernel over `todo` in batches that fit the HBM budget.
+// Plain kernel over `todo`, in the reference driver's launch shape
+// (main.cu:498-735): every read gets a workspace of its own length's size
+// (main.cu:498-520 sizes scratch per read), reads are packed in order into
+// batches, and each batch is one launch of one wavefront per read.  When all
+// workspaces fit the budget that is a single launch; otherwise each of
+// kPlainStreams streams owns an arena of budget / kPlainStreams bytes, the
+// batches go round-robin over the streams, and a stream's next batch reuses
+// its arena after its previous launch (stream order), so launches on
+// different streams overlap.  A read whose workspace alone exceeds an arena
+// gets a batch (and an arena) of its own.
 bool run_plain(RunCtx &cx, RunAcc &acc, const gwf_dgraph_t *dg, const DReads &r, const std::vector<i32> &qlh,
                const std::vector<i32> &todo, int scale, size_t budget) {
-	size_t pos = 0;
-	while (pos < todo.size()) {
-		i32 maxql = 0;
-		size_t end = pos, ws = 0;
-		while (end < todo.size()) {
-			const i32 q = std::max(maxql, qlh[todo[end]]);
-			const size_t wsb = gwf_ws_bytes(caps_for(q, scale));
-			if (end > pos && wsb * (end - pos + 1) > budget) break;
-			maxql = q, ws = wsb, ++end;
+	struct Batch {
+		size_t pos;
+		i32 nb;
+		int stream;
+	};
+	const size_t nt = todo.size();
+	std::vector<u64> off(nt);
+	std::vector<i32> wa(nt);
+	std::vector<size_t> wsb(nt);
+	size_t total = 0;
+	for (size_t j = 0; j < nt; ++j) {
+		const Caps c = caps_for(qlh[todo[j]], scale);
+		wa[j] = c.x;
+		wsb[j] = gwf_ws_bytes(c);
+		total += wsb[j];
+	}
+	const size_t arena = total <= budget ? total : std::max<size_t>(budget / kPlainStreams, 1);
+	std::vector<Batch> bt;
+	size_t need[kPlainStreams] = {};
+	for (size_t pos = 0; pos < nt;) {
+		size_t end = pos, used = 0;
+		while (end < nt && (end == pos || used + wsb[end] <= arena)) {
+			off[end] = used / 8;
+			used += wsb[end++];
 		}
-		const Caps caps = caps_for(maxql, scale);
-		const i32 nb = (i32)(end - pos);
-		ACC(grow(&cx.scratch, &cx.scratch_bytes, ws * nb));
-		ACC(hipMemcpyAsync(cx.idx, todo.data() + pos, 4 * (size_t)nb, hipMemcpyHostToDevice, cx.s));
-		if (!timed_launch(cx, acc, [&] {
-			    return gwf_launch_plain(dg->g, r, cx.idx, nb, caps, cx.scratch, ws, cx.ed, cx.status, cx.cnt, cx.s);
-		    }))
-			return false;
+		const int k = (int)(bt.size() % kPlainStreams);
+		bt.push_back({pos, (i32)(end - pos), k});
+		need[k] = std::max(need[k], used);
 		pos = end;
 	}
-	return true;
+	for (int k = 0; k < kPlainStreams; ++k)
+		if (need[k]) ACC(grow(&cx.pscr[k], &cx.pscr_c[k], need[k]));
+	ACC(hipMemcpyAsync(cx.idx, todo.data(), 4 * nt, hipMemcpyHostToDevice, cx.s));
+	ACC(hipMemcpyAsync(cx.woff, off.data(), 8 * nt, hipMemcpyHostToDevice, cx.s));
+	ACC(hipMemcpyAsync(cx.wa, wa.data(), 4 * nt, hipMemcpyHostToDevice, cx.s));
+	const int used = (int)std::min<size_t>(bt.size(), kPlainStreams);
+	return timed_launch(cx, acc, [&]() -> hipError_t {
+		hipError_t e = hipEventRecord(cx.ev_fork, cx.s);
+		for (int k = 0; k < used && e == hipSuccess; ++k) e = hipStreamWaitEvent(cx.pst[k], cx.ev_fork, 0);
+		for (const Batch &x : bt) {
+			if (e != hipSuccess) break;
+			e = gwf_launch_plain(dg->g, r, cx.idx + x.pos, x.nb, cx.woff + x.pos, cx.wa + x.pos, cx.pscr[x.stream],
+			                     cx.ed, cx.status, cx.cnt, cx.pst[x.stream]);
+		}
+		for (int k = 0; k < used && e == hipSuccess; ++k) {
+			e = hipEventRecord(cx.ev_pj[k], cx.pst[k]);
+			if (e == hipSuccess) e = hipStreamWaitEvent(cx.s, cx.ev_pj[k], 0);
+		}
+		return e;
+	}, nullptr, (int)bt.size());
 }
 
 // Persistent kernel over one host-ordered queue (the autotuner's runs).
@@ -634,6 +718,20 @@ struct BinnedPlan {
 	GwfBeat *beat = nullptr;  // heartbeat records: beat_stride per bin (probe: bin n_bins)
 	i32 beat_stride = 0;
 };
+
+// The longest read length (at most maxql) whose HBM workspace, times wgs
+// workgroups, fits the budget (at least 1: the workspace of a read of length 1).
+i32 bin_ws_ql(i32 maxql, i32 wgs, size_t budget) {
+	auto fits = [&](i32 q) { return gwf_ws_bytes(caps_for(q, 1)) * (size_t)std::max(wgs, 1) <= budget; };
+	if (fits(maxql)) return maxql;
+	i32 lo = 1, hi = maxql;  // fits(lo) or lo == 1; !fits(hi)
+	while (hi - lo > 1) {
+		const i32 mid = lo + (hi - lo) / 2;
+		if (fits(mid)) lo = mid;
+		else hi = mid;
+	}
+	return lo;
+}
 
 hipError_t enqueue_binned(RunCtx &cx, const gwf_dgraph_t *dg, const gwf_dreads_t *dr, const BinnedPlan &p) {
 	hipError_t e;
@@ -690,6 +788,12 @@ bool run_binned(RunCtx &cx, RunAcc &acc, const gwf_dgraph_t *dg, const gwf_dread
 	for (int b = 0; b < t.n_bins; ++b) {
 		p.bb.est_max[b] = t.bin[b].est_max;
 		p.bin[b] = make_cfg(cx, t.bin[b].nw, t.bin[b].lds_states, t.bin[b].merge_runs, maxql);
+		// Every resident workgroup holds one HBM workspace.  Sized for the
+		// longest read they would exceed the budget when a few reads are far
+		// longer than the rest; the workspaces are then sized for the longest
+		// length that fits, and a read whose state outgrows them reports an
+		// overflow and is re-run by the budgeted plain launches below.
+		p.bin[b].hbm = caps_for(bin_ws_ql(maxql, p.bin[b].wgs, budget), 1);
 		const size_t w = gwf_ws_bytes(p.bin[b].hbm) * (size_t)p.bin[b].wgs;
 		ws_sum += w;
 		ws_max = std::max(ws_max, w);
@@ -738,8 +842,12 @@ bool run_binned(RunCtx &cx, RunAcc &acc, const gwf_dgraph_t *dg, const gwf_dread
 		acc.st.kernel_ms += ms;
 		return true;
 	}
-	// everything the captured launches point at or were sized by
-	std::vector<uintptr_t> key = {(uintptr_t)dg, (uintptr_t)dr, (uintptr_t)n, (uintptr_t)tv, (uintptr_t)maxql, (uintptr_t)(v1 + 1),
+	// everything the captured launches point at or were sized by.  The
+	// handles enter by generation (unique per upload) and by the device
+	// memory the launches read, never by host address: a handle freed and
+	// re-created at the same address is a different key.
+	std::vector<uintptr_t> key = {(uintptr_t)dg->gen, (uintptr_t)dr->gen, (uintptr_t)dg->mem, (uintptr_t)dr->mem,
+	                              (uintptr_t)dr->ql_probe, (uintptr_t)n, (uintptr_t)tv, (uintptr_t)maxql, (uintptr_t)(v1 + 1),
 	                              (uintptr_t)p.concurrent, (uintptr_t)p.direct, (uintptr_t)cx.ed, (uintptr_t)cx.status, (uintptr_t)cx.cnt,
 	                              (uintptr_t)cx.scratch, (uintptr_t)cx.pe, (uintptr_t)cx.ps, (uintptr_t)cx.key,
 	                              (uintptr_t)cx.ord};
@@ -748,7 +856,9 @@ bool run_binned(RunCtx &cx, RunAcc &acc, const gwf_dgraph_t *dg, const gwf_dread
 		key.push_back((uintptr_t)p.bin[b].nw);
 		key.push_back((uintptr_t)p.bin[b].wgs);
 		key.push_back((uintptr_t)p.bin[b].lds_bytes);
+		key.push_back((uintptr_t)p.bin[b].hbm.m);
 	}
+	key.push_back((uintptr_t)p.probe.hbm.m);
 	if (!cx.gexec || cx.gkey != key) {
 		if (cx.gexec) (void)hipGraphExecDestroy(cx.gexec);
 		cx.gexec = nullptr;
@@ -801,7 +911,14 @@ int gwf_ed_run_timed(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t v1, uint32_t ma
 	std::vector<int64_t> cnt(GWF_NCOUNT * (size_t)n);
 	size_t free_b = 0, tot_b = 0;
 	if (hipMemGetInfo(&free_b, &tot_b) != hipSuccess) { set_err("hipMemGetInfo failed"); return GWF_ERR_HIP; }
-	const size_t budget = std::min<size_t>((free_b + cx.scratch_bytes) / 2, (size_t)32 << 30);
+	// device bytes the run's workspaces may take (GWF_BUDGET_BYTES overrides
+	// both, for tests): the persistent launches' per-workgroup workspaces, and
+	// the plain launches' per-read ones (the reference's pool takes 90% of
+	// the free memory, main.cu:541; half of it here)
+	const size_t held = ctx_held(cx);
+	const size_t budget = (size_t)env_int("GWF_BUDGET_BYTES", (int64_t)std::min<size_t>((free_b + held) / 2, (size_t)32 << 30));
+	const size_t plain_budget =
+	    (size_t)env_int("GWF_BUDGET_BYTES", (int64_t)std::min<size_t>((free_b + held) / 2, (size_t)128 << 30));
 	std::vector<i32> todo;
 	int scale = 1;
 	bool ok = true;
@@ -826,7 +943,7 @@ int gwf_ed_run_timed(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t v1, uint32_t ma
 	while (ok && rc == GWF_OK && !todo.empty()) {
 		DReads rv = dr->r;
 		rv.v1 = v1;
-		ok = run_plain(cx, acc, dg, rv, dr->ql, todo, scale, budget);
+		ok = run_plain(cx, acc, dg, rv, dr->ql, todo, scale, plain_budget);
 		if (!ok) break;
 		ok = hipMemcpy(status.data(), cx.status, 4 * (size_t)n, hipMemcpyDeviceToHost) == hipSuccess;
 		std::vector<i32> again;
@@ -915,7 +1032,7 @@ int gwf_autotune(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t max_evals, gwf_tune
 		for (i32 q : dr->ql) maxql = std::max(maxql, q);
 		size_t free_b = 0, tot_b = 0;
 		if (hipMemGetInfo(&free_b, &tot_b) != hipSuccess) { set_err("hipMemGetInfo failed"); return GWF_ERR_HIP; }
-		const size_t budget = std::min<size_t>(free_b / 2, (size_t)32 << 30);
+		const size_t budget = (size_t)env_int("GWF_BUDGET_BYTES", (int64_t)std::min<size_t>((free_b + ctx_held(cx)) / 2, (size_t)32 << 30));
 		RunAcc acc;
 		memset(&acc.st, 0, sizeof(acc.st));
 		int evals = 0;
@@ -1054,11 +1171,20 @@ int32_t gwf_ed(const gwf_graph_t *g, int32_t ql, const char *q, int32_t v0, int3
 	{
 		std::lock_guard<std::mutex> lk(g_cache_mu);
 		auto it = g_cache.find(g);
-		if (it != g_cache.end()) dg = it->second;
+		if (it != g_cache.end() && !it->second.matches(g)) {
+			gwf_dgraph_free(it->second.dg);
+			g_cache.erase(it);
+			it = g_cache.end();
+		}
+		if (it != g_cache.end()) dg = it->second.dg;
 		else {
 			int rc = gwf_dgraph_upload(g, &dg);
 			if (rc) return rc;
-			g_cache.emplace(g, dg);
+			CachedGraph c;
+			c.dg = dg;
+			c.n_vtx = g->n_vtx, c.n_arc = g->n_arc;
+			c.len = g->len, c.seq = g->seq, c.arc = g->arc, c.aux = g->aux;
+			g_cache.emplace(g, c);
 		}
 	}
 	const char *qq = q ? q : "";

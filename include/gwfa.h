@@ -170,6 +170,12 @@ int gwf_dreads_upload(int32_t n, const char *seqs, const int64_t *off, const int
                       const int32_t *v0, gwf_dreads_t **out);
 void gwf_dreads_free(gwf_dreads_t *dr);
 
+/* A number unique to this upload within the process (never 0 and never
+ * reused, unlike the handle's address after a free).  Captured launches are
+ * replayed only for the same pair of generations. */
+uint64_t gwf_dgraph_generation(const gwf_dgraph_t *dg);
+uint64_t gwf_dreads_generation(const gwf_dreads_t *dr);
+
 /* Align every read of dr against dg; out[i] (host memory) receives read i's
  * ED.  stats may be NULL.  Reads whose state outgrows the working capacity
  * are re-run with a larger one; GWF_ERR_CAPACITY only past the hard cap. */
