@@ -165,6 +165,8 @@ def hip_lib():
                                           P(C.c_void_p)]
         lib.gwf_dreads_free.argtypes = [C.c_void_p]
         lib.gwf_dreads_free.restype = None
+        lib.gwf_ed_run_noalloc.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.c_uint32, C.c_void_p,
+                                           P(RunStats)]
         lib.gwf_dgraph_generation.argtypes = [C.c_void_p]
         lib.gwf_dgraph_generation.restype = C.c_uint64
         lib.gwf_dreads_generation.argtypes = [C.c_void_p]
@@ -507,6 +509,16 @@ def ed_run(dgraph, dreads, kernel=KERNEL_PLAIN, v1=-1, timed=False):
     if timed:
         d["t_start_ns"], d["t_end_ns"] = t0[:dreads.n], t1[:dreads.n]
     return out[:dreads.n], d
+
+
+def ed_run_noalloc(dgraph, dreads, v1=-1):
+    """gwf_ed_run_noalloc: the plain-kernel run with no device allocation."""
+    n = max(dreads.n, 1)
+    out = np.empty(n, dtype=np.int32)
+    st = RunStats()
+    _check(hip_lib().gwf_ed_run_noalloc(dgraph.ptr, dreads.ptr, v1, 0, out.ctypes.data, C.byref(st)),
+           "gwf_ed_run_noalloc")
+    return out[:dreads.n], st.as_dict()
 
 
 def gwf_ed(graph, q, v0=0, v1=-1):

@@ -47,7 +47,7 @@ def build_host():
     return out
 
 
-HIP_SOURCES = ["gwf_kernels.hip", "gwf_runtime.hip", "gwf_serve.hip"]
+HIP_SOURCES = ["gwf_kernels.hip", "gwf_runtime.hip", "gwf_serve.hip", "gwf_noalloc.hip"]
 
 
 def build_hip():

@@ -196,6 +196,25 @@ int gwf_ed_batch(gwf_dgraph_t *dg, int32_t n, const char *const *q, const int32_
                  const int32_t *v0, int32_t v1, uint32_t max_lag, int32_t kernel,
                  int32_t *out, gwf_run_stats_t *stats);
 
+/* ---- addition: an entry point that allocates nothing --------------------- */
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory where they align (tight serving loops, harnesses that forbid
+ * hipMalloc): gwf_ed_run on the plain kernel with every device buffer the call
+ * needs (EDs, statuses, re-run lists, every read's workspace) taken from one
+ * static device array of GWF_NOALLOC_SCRATCH_BYTES per device, held until the
+ * process ends.  dg and dr are uploaded beforehand (set-up, which allocates).
+ * Reads go in input order in batches of equal workspaces; a read that
+ * outgrows its workspace is re-run with a 4x larger one.  Returns
+ * GWF_ERR_CAPACITY for more than GWF_NOALLOC_MAX_READS reads or a read whose
+ * workspace cannot fit the array.  Synchronous; out is host memory.  Calls
+ * in one process are serialised (they share the array, which keeps the last
+ * call's contents).  stats: kernel_ms is the call's wall time, the state and
+ * character counters are 0. */
+#define GWF_NOALLOC_SCRATCH_BYTES (128ull << 20)
+#define GWF_NOALLOC_MAX_READS (1 << 20)
+int gwf_ed_run_noalloc(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t v1, uint32_t max_lag, int32_t *out,
+                       gwf_run_stats_t *stats);
+
 /* Text of the last error seen by this thread ("" if none). */
 const char *gwf_last_error(void);
 
