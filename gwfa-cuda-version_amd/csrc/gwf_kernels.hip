@@ -699,8 +699,7 @@ __device__ __forceinline__ bool mf_next(QP Q, i32 n_s, XP X, i32 n_x, const u64 
 // sequentially (Q first on ties), keeps the first (furthest) state of each
 // diagonal and tests the history (coalesced intervals Hl/Hh, nh of them)
 // with a pointer that only moves forward.  Kept states go to P: straight from
-// registers when E <= 8, else packed at the front of each range in M and
-// copied after the scan.  Returns their count.
+// registers, in chunks of 8 steps (see below).  Returns their count.
 // QP and XP are lds_cptr or const u64 * (where each list lives).
 template <int NW, typename QP, typename XP>
 __device__ __forceinline__ i32 merge_filter(QP Q, i32 n_s, XP X, i32 n_x, const u64 *Hl, const u64 *Hh, i32 nh, u64 *M,
@@ -735,37 +734,42 @@ __device__ __forceinline__ i32 merge_filter(QP Q, i32 n_s, XP X, i32 n_x, const 
 			else c = mid;
 		}
 	}
-	i32 n_new;
-	if (E <= CH) {
-		u64 kv[CH];
-		u32 km = 0;
+	// Chunks of CH merge steps in registers.  The last chunk's kept states
+	// stay in registers until the scan has given their place in P; earlier
+	// chunks (E > CH) are stored to M after their steps, packed at the front
+	// of the thread's range (on this GPU a wait for a load also waits for the
+	// stores issued before it, so a store between every two steps would add
+	// a store round trip to each step).
+	i32 n_new, in_m = 0;
+	u64 kv[CH];
+	u32 km = 0;
+	for (i32 o = o0; o < o1; o += CH) {
+		if (km) {  // the previous chunk was not the last one
+#pragma unroll
+			for (i32 u = 0; u < CH; ++u)
+				if ((km >> u) & 1) M[o0 + in_m + __popc(km & ((1u << u) - 1))] = kv[u];
+			in_m += __popc(km);
+		}
+		km = 0;
 #pragma unroll
 		for (i32 u = 0; u < CH; ++u) {
 			u64 x = 0;
-			if (o0 + u < o1 && mf_next(Q, n_s, X, n_x, Hl, Hh, nh, i, jx, qh, xh, prev_dg, h, x)) km |= 1u << u;
+			if (o + u < o1 && mf_next(Q, n_s, X, n_x, Hl, Hh, nh, i, jx, qh, xh, prev_dg, h, x)) km |= 1u << u;
 			kv[u] = x;
 		}
-		const i32 at = b.scan(__popc(km), &n_new);
-#pragma unroll
-		for (i32 u = 0; u < CH; ++u)
-			if ((km >> u) & 1) P[at + __popc(km & ((1u << u) - 1))] = kv[u];
-	} else {
-		i32 kept = 0;
-		for (i32 o = o0; o < o1; ++o) {
-			u64 x;
-			if (mf_next(Q, n_s, X, n_x, Hl, Hh, nh, i, jx, qh, xh, prev_dg, h, x)) M[o0 + kept++] = x;
-		}
-		// one scan of the per-thread counts; each thread copies its own kept states
-		const i32 at = b.scan(kept, &n_new);
-		for (i32 e0 = 0; e0 < kept; e0 += 4) {
-			u64 t[4];
-#pragma unroll
-			for (i32 r = 0; r < 4; ++r) t[r] = e0 + r < kept ? M[o0 + e0 + r] : 0;
-#pragma unroll
-			for (i32 r = 0; r < 4; ++r)
-				if (e0 + r < kept) P[at + e0 + r] = t[r];
-		}
 	}
+	const i32 at = b.scan(in_m + __popc(km), &n_new);
+	for (i32 e0 = 0; e0 < in_m; e0 += 4) {
+		u64 t[4];
+#pragma unroll
+		for (i32 r = 0; r < 4; ++r) t[r] = e0 + r < in_m ? M[o0 + e0 + r] : 0;
+#pragma unroll
+		for (i32 r = 0; r < 4; ++r)
+			if (e0 + r < in_m) P[at + e0 + r] = t[r];
+	}
+#pragma unroll
+	for (i32 u = 0; u < CH; ++u)
+		if ((km >> u) & 1) P[at + in_m + __popc(km & ((1u << u) - 1))] = kv[u];
 	b.sync();
 	return n_new;
 }
@@ -1051,9 +1055,13 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 			uint2 vi[S];
 #pragma unroll
 			for (i32 s = 0; s < S; ++s) key[s] = j0 + s < n_a ? w.P[j0 + s] : EMPTY_KEY;
+			// loads are unconditional (from a valid address when the lane has no
+			// state): loads under a branch are waited for one after the other
 #pragma unroll
-			for (i32 s = 0; s < S; ++s)
-				vi[s] = key[s] != EMPTY_KEY ? *(const uint2 *)&g.vinfo[key_v(key[s])] : make_uint2(0, 0);
+			for (i32 s = 0; s < S; ++s) {
+				const uint2 t = *(const uint2 *)&g.vinfo[key[s] != EMPTY_KEY ? key_v(key[s]) : 0u];
+				vi[s] = key[s] != EMPTY_KEY ? t : make_uint2(0, 0);
+			}
 			i32 kk[S], mk8[S];
 			u64 x8[S];
 #pragma unroll
@@ -1062,7 +1070,9 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 				kk[s] = key_k(key[s]);
 				mk8[s] = (ql - d < vl ? ql - d : vl) - 1;
 				const bool go = key[s] != EMPTY_KEY && mk8[s] > kk[s];
-				x8[s] = go ? load8u(g.seq + vi[s].x + kk[s] + 1) ^ load8u(q + d + kk[s] + 1) : 0;
+				const u64 gx = load8u(g.seq + (go ? vi[s].x + kk[s] + 1 : 0u));
+				const u64 qx = load8u(q + (go ? d + kk[s] + 1 : 0));
+				x8[s] = go ? gx ^ qx : 0;
 			}
 			bool more[S];
 #pragma unroll
@@ -1266,13 +1276,22 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 	i32 lds_used = 0;     // HBM mode: elements of the LDS scratch holding the sorted X
 	bool x_lds = false;  // HBM mode: the sorted X is in the LDS scratch
 	if constexpr (HBM) {
-		if (2 * n_x <= w.sort_n - hist_res - q_res) {  // stage X in LDS, sort it there and merge from there
+		const i32 free_n = w.sort_n - hist_res - q_res;  // LDS scratch below the staged history and Q
+		i32 free_cap = 128;
+		while (free_cap * 2 <= free_n) free_cap <<= 1;
+		if (2 * n_x <= free_n) {  // stage X in LDS, sort it there and merge from there
 			stage(w.sort_lds, w.X, n_x, b);
 			b.sync();
 			sort_runs(w.sort_lds, n_x, w.sort_lds + n_x, w.mw, b);
 			Xs = w.sort_lds + n_x;
 			lds_used = 2 * n_x;
 			x_lds = true;
+		} else if ((hist_res || q_res) && free_cap >= 256 && free_cap <= free_n) {
+			// X sorted in HBM through the free part of the scratch, so that the
+			// history and Q staged above it stay there for the merge (which
+			// then takes most of its inputs from LDS)
+			sort_hbm(w.X, n_x, w.sort_lds, free_cap, w.sort_tmp, b);
+			Xs = w.X;
 		} else {
 			if (hist_res) {  // the sort takes the whole scratch: the filter reads the history from HBM
 				b.sync();
@@ -1303,11 +1322,12 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 			Hl = hl, Hh = hh;
 			used += 2 * nh2;
 		}
-		const bool q_lds = x_lds && (q_res || used + n_s <= avail);
+		const bool q_lds = q_res || used + n_s <= avail;
 		u64 *Ql = q_res ? w.sort_lds + w.sort_n - hist_res - q_res : w.sort_lds + used;
 		if (q_lds && !q_res) stage(Ql, w.Q, n_s, b);
 		b.sync();
-		if (q_lds) n_new = merge_filter<NW>((lds_cptr)Ql, n_s, (lds_cptr)Xs, n_x, Hl, Hh, nh2, M, w.P, b);
+		if (q_lds && x_lds) n_new = merge_filter<NW>((lds_cptr)Ql, n_s, (lds_cptr)Xs, n_x, Hl, Hh, nh2, M, w.P, b);
+		else if (q_lds) n_new = merge_filter<NW>((lds_cptr)Ql, n_s, Xs, n_x, Hl, Hh, nh2, M, w.P, b);
 		else if (x_lds) n_new = merge_filter<NW>((const u64 *)w.Q, n_s, (lds_cptr)Xs, n_x, Hl, Hh, nh2, M, w.P, b);
 		else n_new = merge_filter<NW>((const u64 *)w.Q, n_s, Xs, n_x, Hl, Hh, nh2, M, w.P, b);
 	} else {
