@@ -31,6 +31,8 @@ struct DGraph {
 	const uint4 *arc;
 	u32 n_vtx;
 	u32 total_len;  // sum of vertex lengths
+	u32 mixed;      // 1: some vertex has in-arcs with different overlaps (per-read visited set)
+	u32 pad;
 };
 
 struct DReads {
@@ -49,7 +51,17 @@ struct Caps {
 	i32 f;   // closure frontier, each of two buffers
 	i32 t;   // exits recorded in one step
 	i32 h;   // forbidden-interval history, each of two buffers
+	i32 v;   // per-read visited set, slots (a power of two; 0: none, the per-step LDS set is used)
 };
+
+// Slots of the per-read visited set of a workspace whose other capacities
+// derive from a (Caps {2a, a, a, a, a}): graphs with mixed in-overlaps only.
+__host__ __device__ inline i32 gwf_vis_slots(i32 a, bool mixed) {
+	if (!mixed) return 0;
+	i32 v = 1024;
+	while (v < 4 * a && v < (1 << 27)) v <<= 1;
+	return v;
+}
 
 // Per-read counters written by the kernels: score steps, wavefront states
 // processed, characters compared, device wall-clock ticks at start and end,
@@ -154,7 +166,6 @@ struct SrvEntry {
 	Caps hc;       // HBM workspace capacities
 	u32 res_slot;
 	i32 v1;        // end vertex (< 0: anywhere)
-	i32 pad;
 };
 struct SrvResult {
 	i32 ed, status;

@@ -794,6 +794,26 @@ __device__ __forceinline__ bool vis_insert(u64 *vis, u64 key) {
 	return true;
 }
 
+// Slot of key K in a read's visited set (w.VK/VS/vmask), inserted when absent
+// if `insert`; -1 when the probe window is exhausted (the set counts as full).
+template <class BF>
+__device__ __forceinline__ i32 vs_slot(const BF &w, u64 K, bool insert) {
+	u32 h = vis_hash(K) & w.vmask;
+	for (int p = 0; p < 32; ++p) {
+		const u64 old = insert ? atomicCAS((unsigned long long *)&w.VK[h], (unsigned long long)EMPTY_KEY,
+		                                   (unsigned long long)K)
+		                       : __hip_atomic_load(&w.VK[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		if (old == K) return (i32)h;
+		if (old == EMPTY_KEY) {
+			if (!insert) return -1;
+			atomicAdd(w.vfill, 1);
+			return (i32)h;
+		}
+		h = (h + 1) & w.vmask;
+	}
+	return -1;
+}
+
 // ------------------------------------------------------------ working arrays
 struct Bufs {
 	u64 *P, *Q, *X, *F0, *F1, *T;
@@ -803,11 +823,17 @@ struct Bufs {
 	i32 sort_n;       // its size in elements
 	i32 mw;           // LDS sort: runs of 64 merged in one rank pass when at most mw of them
 	u64 *sort_tmp;    // HBM mode: merge temp (max(x, t) elements)
+	// per-read visited set (graphs with mixed in-overlaps; VK == nullptr:
+	// none): keys, and per key the stamp of the candidate that owns it
+	u64 *VK, *VS;
+	u32 vmask;        // slots - 1
+	i32 *vfill;       // LDS: slots taken
 	Caps c;
 };
 
 __host__ __device__ inline size_t ws_elems(const Caps &c) {
-	return 2 * (size_t)c.m + c.x + 2 * (size_t)c.f + c.t + 4 * (size_t)c.h + (size_t)(c.x > c.t ? c.x : c.t) + 8;
+	return 2 * (size_t)c.m + c.x + 2 * (size_t)c.f + c.t + 4 * (size_t)c.h + (size_t)(c.x > c.t ? c.x : c.t) + 8 +
+	       2 * (size_t)c.v;
 }
 size_t gwf_ws_bytes(const Caps &c) { return ((ws_elems(c) * 8 + 255) / 256) * 256; }
 // elements between equal plain-kernel workspaces (gwf_ws_bytes / 8)
@@ -831,6 +857,11 @@ __device__ inline Bufs carve(u64 *p, const Caps &c, bool with_tmp) {
 	w.H1lo = p; p += c.h;
 	w.H1hi = p; p += c.h;
 	w.sort_tmp = with_tmp ? p : nullptr;
+	p += (c.x > c.t ? c.x : c.t) + 8;
+	w.VK = with_tmp && c.v ? p : nullptr;
+	w.VS = p + c.v;
+	w.vmask = c.v ? (u32)c.v - 1 : 0;
+	w.vfill = nullptr;
 	w.sort_lds = nullptr;
 	w.sort_n = 0;
 	w.mw = 8;
@@ -895,6 +926,118 @@ __device__ __forceinline__ i32 hist_lds(const Bufs &w, const B &b, const u64 *Hc
 	return nh2;
 }
 
+// The crossing part of one closure level on a graph whose vertices have
+// in-arcs with different overlaps.  There a visited key (w, i+1) admits a
+// different entry depending on which arc claims it first, so the reference's
+// order is followed exactly (gwf-ed.cu:1106-1227: a FIFO queue, each popped
+// state's arcs in CSR order, one visited set for the whole read): pass 1
+// ranks every (frontier state, arc) candidate in that order and lets it claim
+// its key in the read's set with the stamp (level, rank), the lowest stamp
+// staying; pass 2 keeps the candidates that own their key and writes the
+// next level's entries state by state, arc by arc.  Returns true when the
+// set is full (the caller reports a spill; the read is re-run larger).
+template <class B>
+__device__ __forceinline__ bool closure_mixed(const DGraph &g, const uint8_t *q, i32 ql, const Bufs &w, const B &b,
+                                              i32 v1, i32 n_f, bool closure_origin, u64 &lvl, i32 &n_na, i32 &n_x,
+                                              i32 &n_t) {
+	constexpr i32 NT = B::NT;
+	const i32 tid = (i32)b.tid();
+	const Caps &cap = w.c;
+	const u64 lv = ++lvl;
+	bool full = false;
+	i32 rank_base = 0;
+	for (i32 base = 0; base < n_f; base += NT) {
+		const i32 j = base + tid;
+		const bool act = j < n_f;
+		const u64 key = act ? w.F0[j] : 0;
+		const i32 i = key_d(key) + key_k(key);
+		const uint4 vi = act ? g.vinfo[key_v(key)] : make_uint4(0, 0, 0, 0);
+		const i32 deg = act && i + 1 != ql ? (i32)vi.w : 0;  // a query end claims no key
+		i32 tot;
+		const i32 r0 = rank_base + b.scan(deg, &tot);
+		rank_base += tot;
+		for (i32 e = 0; e < deg && !full; ++e) {
+			const i32 sl = vs_slot(w, (u64)g.arc[vi.z + e].x << 32 | (u32)(i + 1), true);
+			if (sl < 0) full = true;
+			else atomicMin((unsigned long long *)&w.VS[sl], (unsigned long long)(lv << 24 | (u64)(r0 + e)));
+		}
+	}
+	// every claim is in memory before any candidate reads its key's stamp
+	__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+	b.sync();
+	if (b.any(full || rank_base >= (1 << 24) || *w.vfill > (i32)(w.vmask >> 1))) return true;
+	rank_base = 0;
+	for (i32 base = 0; base < n_f; base += NT) {
+		const i32 j = base + tid;
+		const bool act = j < n_f;
+		const u64 key = act ? w.F0[j] : 0;
+		const u32 v = key_v(key);
+		const i32 d = key_d(key), k = key_k(key), i = d + k;
+		const uint4 vi = act ? g.vinfo[v] : make_uint4(0, 0, 0, 0);
+		const bool rec = act && closure_origin;
+		// query end at a vertex end (only with v1 >= 0): delete into every next
+		// vertex, nothing else (gwf-ed.cu:1244-1254)
+		const bool qe = act && i + 1 == ql;
+		const uint8_t c = act ? q[i + 1] : 0;
+		const i32 degall = act ? (i32)vi.w : 0;
+		i32 tot;
+		const i32 r0 = rank_base + b.scan(qe ? 0 : degall, &tot);
+		rank_base += tot;
+		auto owns = [&](u32 wv, i32 e) {
+			const i32 sl = vs_slot(w, (u64)wv << 32 | (u32)(i + 1), false);
+			return sl >= 0 && __hip_atomic_load(&w.VS[sl], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+			                      (lv << 24 | (u64)(r0 + e));
+		};
+		i32 na = 0, nx = 0, n_ext = 0;
+		for (i32 e = 0; e < degall; ++e) {
+			const uint4 ar = g.arc[vi.z + e];
+			if (qe) {
+				nx += 1;
+				continue;
+			}
+			const bool m = (uint8_t)(ar.y >> 24) == c;
+			n_ext += m;
+			const bool own = owns(ar.x, e);
+			na += own && m;
+			nx += own && !m ? 2 : 0;
+		}
+		const bool pi = rec && !qe && (degall == 0 || n_ext != degall);
+		const bool t2 = rec && !qe && v1 >= 0;
+		i32 ta, tx, tt;
+		i32 sa = n_na + b.scan(na, &ta);
+		i32 sx = n_x + b.scan(nx + (pi ? 1 : 0), &tx);
+		const i32 st = n_t + b.scan(t2 ? 1 : 0, &tt);
+		n_na += ta;
+		n_x += tx;
+		n_t += tt;
+		for (i32 e = 0; e < degall; ++e) {
+			const uint4 ar = g.arc[vi.z + e];
+			const u32 wv = ar.x;
+			const i32 ol = (i32)(ar.y & 0xFFFFFFu);
+			if (qe) {
+				if (sx < cap.x) w.X[sx] = mk(wv, i - ol, ol);
+				++sx;
+				continue;
+			}
+			const bool m = (uint8_t)(ar.y >> 24) == c;
+			if (!owns(wv, e)) continue;
+			if (m) {
+				if (sa < cap.f) w.F1[sa] = mk(wv, i + 1 - ol, ol);
+				++sa;
+			} else {
+				if (sx + 1 < cap.x) {
+					w.X[sx] = mk(wv, i - ol, ol);
+					w.X[sx + 1] = mk(wv, i + 1 - ol, ol);
+				}
+				sx += 2;
+			}
+		}
+		if (pi && sx < cap.x) w.X[sx] = mk(v, d + 1, k);
+		if (t2 && st < cap.t) w.T[st] = diag_id(v, d);
+	}
+	return false;
+}
+
 // P3, the vertex-crossing closure of one step, by frontier levels: a
 // frontier state at a vertex end follows each out-arc (one 16-byte record:
 // target, overlap, first base); a match enters the target and is extended
@@ -904,9 +1047,9 @@ __device__ __forceinline__ i32 hist_lds(const Bufs &w, const B &b, const u64 *Hc
 // position) once.  Appends successors to X and (v1 >= 0) exits to T;
 // returns STEP_NEXT, STEP_FOUND or STEP_SPILL.  B: the whole workgroup
 // (Blk) or one wave (WaveB).
-template <class B>
+template <bool MIXED, class B>
 __device__ __forceinline__ int closure(const DGraph &g, const uint8_t *q, i32 ql, const Bufs &w, const B &b, u64 *vis,
-                                       i32 v1, i32 n_f, i32 &n_x, i32 &n_t, int64_t &chars) {
+                                       i32 v1, i32 n_f, i32 &n_x, i32 &n_t, int64_t &chars, u64 &lvl) {
 	constexpr i32 NT = B::NT;
 	const i32 tid = (i32)b.tid();
 	const Caps &cap = w.c;
@@ -916,7 +1059,10 @@ __device__ __forceinline__ int closure(const DGraph &g, const uint8_t *q, i32 ql
 	bool closure_origin = false;
 	while (n_f > 0) {
 		i32 n_na = 0;
-		for (i32 base = 0; base < n_f; base += NT) {
+		if constexpr (MIXED) {  // mixed in-overlaps: the reference's order and one visited set per read
+			if (closure_mixed(g, q, ql, w, b, v1, n_f, closure_origin, lvl, n_na, n_x, n_t)) return STEP_SPILL;
+		}
+		for (i32 base = 0; !MIXED && base < n_f; base += NT) {
 			const i32 j = base + tid;
 			const bool act = j < n_f;
 			const u64 key = act ? w.F0[j] : 0;
@@ -1014,9 +1160,9 @@ __device__ __forceinline__ int closure(const DGraph &g, const uint8_t *q, i32 ql
 // updated history in H[hs ^ 1].  On STEP_SPILL nothing the next attempt needs
 // was modified (P holds the same states, extended in place, which is
 // idempotent).  HBM selects how sorts are staged.
-template <int NW, bool HBM>
+template <int NW, bool HBM, bool MIXED>
 __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 ql, const Bufs &w, const Blk<NW> &b,
-                                        u64 *vis, i32 v1, i32 &n_a, i32 &n_h, i32 &hs, int64_t &chars) {
+                                        u64 *vis, i32 v1, i32 &n_a, i32 &n_h, i32 &hs, int64_t &chars, u64 &lvl) {
 	constexpr i32 NT = Blk<NW>::NT;
 	const i32 tid = (i32)b.tid();
 	const Caps &cap = w.c;
@@ -1196,7 +1342,7 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 			const WaveB wb;
 			if (b.wid() == 0) {
 				i32 nx = 0, nt = n_t;
-				const int rc = closure(g, q, ql, w, wb, vis, v1, n_f, nx, nt, chars);
+				const int rc = closure<MIXED>(g, q, ql, w, wb, vis, v1, n_f, nx, nt, chars, lvl);
 				if (lane_id() == 0) b.xch[0] = rc, b.xch[1] = nx;
 			} else if (b.wid() == 1) {
 				const i32 h2 = hist_lds(w, wb, Hcur_lo, Hcur_hi, n_h, n_t, Hlo, Hhi);
@@ -1215,7 +1361,7 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 		}
 	}
 	if (!p5_done) {
-		const int rc = closure(g, q, ql, w, b, vis, v1, n_f, n_x, n_t, chars);
+		const int rc = closure<MIXED>(g, q, ql, w, b, vis, v1, n_f, n_x, n_t, chars, lvl);
 		if (rc != STEP_NEXT) return rc;
 	}
 
@@ -1400,7 +1546,7 @@ __device__ __forceinline__ void beat_put(GwfBeat *bt, i32 rid, i32 s, i32 n_a, i
 
 // Align one read: LDS-resident while it fits (USE_LDS), then in HBM.  beat
 // (may be null) receives a progress record every GWF_BEAT_EVERY steps.
-template <int NW, bool USE_LDS>
+template <int NW, bool USE_LDS, bool MIXED>
 __device__ __forceinline__ ReadOut align_read(const DGraph &g, const uint8_t *q, i32 ql, u32 v0, i32 v1, const Bufs &lds,
                               const Bufs &hbm, const Blk<NW> &b, u64 *vis, GwfBeat *beat, i32 rid) {
 	ReadOut o;
@@ -1408,6 +1554,12 @@ __device__ __forceinline__ ReadOut align_read(const DGraph &g, const uint8_t *q,
 	i32 n_a = 1, n_h = 0, hs = 0;
 	int64_t chars = 0;
 	bool in_lds = USE_LDS;
+	u64 lvl = 0;  // closure levels run so far (the per-read visited set's stamps)
+	if constexpr (MIXED) {  // mixed in-overlaps: a fresh visited set for the read
+		for (u32 t = b.tid(); t <= hbm.vmask; t += Blk<NW>::NT) hbm.VK[t] = EMPTY_KEY, hbm.VS[t] = ~0ull;
+		if (b.tid() == 0) *hbm.vfill = 0;
+		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+	}
 	if (b.tid() == 0) (in_lds ? lds.P : hbm.P)[0] = mk(v0, 0, -1);
 	b.sync();
 	for (i32 s = 0;; ++s) {
@@ -1419,15 +1571,18 @@ __device__ __forceinline__ ReadOut align_read(const DGraph &g, const uint8_t *q,
 		int rc = STEP_SPILL;
 		if constexpr (USE_LDS) {
 			if (in_lds) {
-				rc = gwf_step<NW, false>(g, q, ql, lds, b, vis, v1, n_a, n_h, hs, chars);
+				const u64 lvl0 = lvl;
+				rc = gwf_step<NW, false, MIXED>(g, q, ql, lds, b, vis, v1, n_a, n_h, hs, chars, lvl);
 				if (rc == STEP_SPILL) {  // the step input is intact: move it to HBM and redo the step there
+					// (with the same closure stamps: its claims repeat those of the attempt)
+					lvl = lvl0;
 					spill_state(lds, hbm, n_a, n_h, hs, b);
 					in_lds = false;
 					o.spill_step = s;
 				}
 			}
 		}
-		if (rc == STEP_SPILL) rc = gwf_step<NW, true>(g, q, ql, hbm, b, vis, v1, n_a, n_h, hs, chars);
+		if (rc == STEP_SPILL) rc = gwf_step<NW, true, MIXED>(g, q, ql, hbm, b, vis, v1, n_a, n_h, hs, chars, lvl);
 		if (rc == STEP_SPILL) { o.status = GWF_ST_OVERFLOW; break; }
 		if (rc == STEP_FOUND) { o.ed = s; break; }
 		if (rc == STEP_EMPTY) { o.ed = -1; break; }
@@ -1469,6 +1624,7 @@ __device__ __forceinline__ void write_out(const ReadOut &o, i32 r, u64 t0, i32 *
 // aligns read read_idx[j] (read_idx == nullptr: read base + j) in a workspace
 // of its own size (ws_off[j], ws_a[j]), or, with ws_off == nullptr, in slot j
 // of equal workspaces of capacity a_uniform.
+template <bool MIXED>
 __global__ __launch_bounds__(WAVE) void gwf_plain_kernel(DGraph g, DReads rd, const i32 *read_idx, i32 base, i32 n,
                                                          const u64 *ws_off, const i32 *ws_a, i32 a_uniform,
                                                          u64 *scratch, i32 *out_ed, i32 *out_status,
@@ -1486,12 +1642,15 @@ __global__ __launch_bounds__(WAVE) void gwf_plain_kernel(DGraph g, DReads rd, co
 	Caps caps;
 	caps.m = 2 * a;
 	caps.x = caps.f = caps.t = caps.h = a;
+	caps.v = gwf_vis_slots(a, MIXED);
 	Bufs w = carve(scratch + (ws_off ? ws_off[blk] : (u64)blk * plain_stride(caps)), caps, true);
+	__shared__ i32 vfill;
+	w.vfill = &vfill;
 	w.sort_lds = lds_sort;
 	w.sort_cap = SORT_CHUNK;
 	w.sort_n = SORT_CHUNK;
 	const u64 t0 = wall_clock64();
-	const ReadOut o = align_read<1, false>(g, rd.seq + rd.off[r], rd.ql[r], (u32)rd.v0[r], rd.v1, w, w, b, vis, nullptr, r);
+	const ReadOut o = align_read<1, false, MIXED>(g, rd.seq + rd.off[r], rd.ql[r], (u32)rd.v0[r], rd.v1, w, w, b, vis, nullptr, r);
 	write_out<1>(o, r, t0, out_ed, out_status, out_counts, b);
 }
 
@@ -1516,7 +1675,7 @@ __device__ __forceinline__ i32 arena_sort_cap(size_t le) {
 // The step is latency-bound: resident waves matter more than a few spilled
 // registers.  Capping registers at 5 waves per SIMD (96 VGPRs, a few dozen
 // bytes per lane spilled) measured 9-15% faster than the uncapped 4 waves.
-template <int NW>
+template <int NW, bool MIXED>
 __global__ __launch_bounds__(WAVE *NW) __attribute__((amdgpu_waves_per_eu(5))) void gwf_persistent_kernel(
     DGraph g, DReads rd, const i32 *order,
                                                                  const i32 *bin_off, i32 bin, i32 *heads, Caps lc,
@@ -1533,6 +1692,9 @@ __global__ __launch_bounds__(WAVE *NW) __attribute__((amdgpu_waves_per_eu(5))) v
 	Bufs lb = carve(arena, lc, false);
 	lb.mw = mw;
 	Bufs hb = carve(scratch + (size_t)blockIdx.x * ws_elems_per_wg, hc, true);
+	__shared__ i32 vfill;
+	hb.vfill = &vfill;
+	lb.VK = hb.VK, lb.VS = hb.VS, lb.vmask = hb.vmask, lb.vfill = hb.vfill;  // one visited set in both modes
 	const i32 q_lo = bin_off[bin], q_n = bin_off[bin + 1] - q_lo;
 	hb.sort_lds = arena;  // the LDS arena is free once a read has spilled
 	hb.sort_cap = arena_sort_cap((size_t)arena_n);
@@ -1547,7 +1709,7 @@ __global__ __launch_bounds__(WAVE *NW) __attribute__((amdgpu_waves_per_eu(5))) v
 		if (qi >= q_n) break;
 		const i32 r = order ? order[q_lo + qi] : q_lo + qi;
 		const u64 t0 = wall_clock64();
-		const ReadOut o = align_read<NW, true>(g, rd.seq + rd.off[r], rd.ql[r], (u32)rd.v0[r], rd.v1, lb, hb, b, vis, beat, r);
+		const ReadOut o = align_read<NW, true, MIXED>(g, rd.seq + rd.off[r], rd.ql[r], (u32)rd.v0[r], rd.v1, lb, hb, b, vis, beat, r);
 		write_out<NW>(o, r, t0, out_ed, out_status, out_counts, b);
 		b.sync();
 	}
@@ -1570,7 +1732,7 @@ __device__ __forceinline__ u64 host_ld(const u64 *p) {
 	return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-template <int NW>
+template <int NW, bool MIXED>
 __global__ __launch_bounds__(WAVE *NW) __attribute__((amdgpu_waves_per_eu(4))) void gwf_serve_kernel(SrvArgs a) {
 	extern __shared__ u64 arena[];
 	__shared__ u64 vis[VIS_SLOTS];
@@ -1580,6 +1742,7 @@ __global__ __launch_bounds__(WAVE *NW) __attribute__((amdgpu_waves_per_eu(4))) v
 	__shared__ int64_t s_seq, s_claim;
 	__shared__ i32 s_polls;
 	__shared__ SrvEntry s_e;
+	__shared__ i32 vfill;
 	constexpr i32 NT = Blk<NW>::NT;
 	static_assert(sizeof(SrvEntry) == 64, "SrvEntry is eight u64 words");
 	Blk<NW> b{red, redu, xch};
@@ -1662,6 +1825,7 @@ __global__ __launch_bounds__(WAVE *NW) __attribute__((amdgpu_waves_per_eu(4))) v
 		e.hc.f = __builtin_amdgcn_readfirstlane(s_e.hc.f);
 		e.hc.t = __builtin_amdgcn_readfirstlane(s_e.hc.t);
 		e.hc.h = __builtin_amdgcn_readfirstlane(s_e.hc.h);
+		e.hc.v = __builtin_amdgcn_readfirstlane(s_e.hc.v);
 		e.res_slot = (u32)__builtin_amdgcn_readfirstlane((i32)s_e.res_slot);
 		uint8_t *dq = a.pool + e.d_off;
 		{
@@ -1672,12 +1836,14 @@ __global__ __launch_bounds__(WAVE *NW) __attribute__((amdgpu_waves_per_eu(4))) v
 		b.sync();
 		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 		Bufs hb = carve((u64 *)(a.pool + e.d_off + e.ws_off), e.hc, true);
+		hb.vfill = &vfill;
+		lb.VK = hb.VK, lb.VS = hb.VS, lb.vmask = hb.vmask, lb.vfill = hb.vfill;  // one visited set in both modes
 		hb.sort_lds = arena;
 		hb.sort_cap = sort_cap;
 		hb.sort_n = (i32)lds_elems(a.lc);
 		hb.mw = a.mw;
 		const u64 t0 = wall_clock64();
-		const ReadOut o = align_read<NW, true>(a.g, dq, e.ql, (u32)e.v0, e.v1, lb, hb, b, vis, beat, (i32)seq);
+		const ReadOut o = align_read<NW, true, MIXED>(a.g, dq, e.ql, (u32)e.v0, e.v1, lb, hb, b, vis, beat, (i32)seq);
 		int64_t c = o.chars;
 #pragma unroll
 		for (int s = 32; s > 0; s >>= 1) c += __shfl_xor(c, s);
@@ -1786,9 +1952,12 @@ hipError_t gwf_launch_fill(i32 *p, i32 n, i32 v, hipStream_t st) {
 hipError_t gwf_kernels_init() {
 	const int mx = (int)gwf_max_lds_bytes();
 	hipError_t e = hipSuccess;
-	const void *ks[] = {(const void *)gwf_persistent_kernel<1>, (const void *)gwf_persistent_kernel<2>,
-	                    (const void *)gwf_persistent_kernel<4>, (const void *)gwf_serve_kernel<1>,
-	                    (const void *)gwf_serve_kernel<2>, (const void *)gwf_serve_kernel<4>};
+	const void *ks[] = {(const void *)gwf_persistent_kernel<1, false>, (const void *)gwf_persistent_kernel<2, false>,
+	                    (const void *)gwf_persistent_kernel<4, false>, (const void *)gwf_serve_kernel<1, false>,
+	                    (const void *)gwf_serve_kernel<2, false>, (const void *)gwf_serve_kernel<4, false>,
+	                    (const void *)gwf_persistent_kernel<1, true>, (const void *)gwf_persistent_kernel<2, true>,
+	                    (const void *)gwf_persistent_kernel<4, true>, (const void *)gwf_serve_kernel<1, true>,
+	                    (const void *)gwf_serve_kernel<2, true>, (const void *)gwf_serve_kernel<4, true>};
 	for (const void *k : ks)
 		if (e == hipSuccess) e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
 	return e;
@@ -1812,16 +1981,24 @@ hipError_t gwf_launch_plain(const DGraph &g, const DReads &r, const i32 *read_id
                             const i32 *ws_a, u64 *scratch, i32 *out_ed, i32 *out_status, int64_t *out_counts,
                             hipStream_t st) {
 	if (n <= 0) return hipSuccess;
-	hipLaunchKernelGGL(gwf_plain_kernel, dim3(n), dim3(WAVE), 0, st, g, r, read_idx, 0, n, ws_off, ws_a, 0, scratch,
-	                   out_ed, out_status, out_counts);
+	if (g.mixed)
+		hipLaunchKernelGGL(gwf_plain_kernel<true>, dim3(n), dim3(WAVE), 0, st, g, r, read_idx, 0, n, ws_off, ws_a, 0,
+		                   scratch, out_ed, out_status, out_counts);
+	else
+		hipLaunchKernelGGL(gwf_plain_kernel<false>, dim3(n), dim3(WAVE), 0, st, g, r, read_idx, 0, n, ws_off, ws_a, 0,
+		                   scratch, out_ed, out_status, out_counts);
 	return hipGetLastError();
 }
 
 hipError_t gwf_launch_plain_uniform(const DGraph &g, const DReads &r, const i32 *read_idx, i32 base, i32 n, i32 a,
                                     u64 *scratch, i32 *out_ed, i32 *out_status, int64_t *out_counts, hipStream_t st) {
 	if (n <= 0) return hipSuccess;
-	hipLaunchKernelGGL(gwf_plain_kernel, dim3(n), dim3(WAVE), 0, st, g, r, read_idx, base, n, nullptr, nullptr, a,
-	                   scratch, out_ed, out_status, out_counts);
+	if (g.mixed)
+		hipLaunchKernelGGL(gwf_plain_kernel<true>, dim3(n), dim3(WAVE), 0, st, g, r, read_idx, base, n, nullptr, nullptr,
+		                   a, scratch, out_ed, out_status, out_counts);
+	else
+		hipLaunchKernelGGL(gwf_plain_kernel<false>, dim3(n), dim3(WAVE), 0, st, g, r, read_idx, base, n, nullptr, nullptr,
+		                   a, scratch, out_ed, out_status, out_counts);
 	return hipGetLastError();
 }
 
@@ -1829,7 +2006,8 @@ template <int NW>
 static hipError_t launch_persist(const DGraph &g, const DReads &r, const i32 *order, const i32 *bin_off, i32 bin,
                                  i32 *heads, const PersistCfg &cfg, u64 *scratch, size_t ws_bytes, i32 *out_ed,
                                  i32 *out_status, int64_t *out_counts, GwfBeat *beat, hipStream_t st) {
-	hipLaunchKernelGGL(gwf_persistent_kernel<NW>, dim3(cfg.wgs), dim3(WAVE * NW), cfg.lds_bytes, st, g, r, order,
+	auto k = g.mixed ? gwf_persistent_kernel<NW, true> : gwf_persistent_kernel<NW, false>;
+	hipLaunchKernelGGL(k, dim3(cfg.wgs), dim3(WAVE * NW), cfg.lds_bytes, st, g, r, order,
 	                   bin_off, bin, heads, cfg.lds, cfg.hbm, cfg.mw, (i32)(cfg.lds_bytes / 8), scratch, (u64)(ws_bytes / 8),
 	                   out_ed, out_status,
 	                   out_counts, beat);
@@ -1838,7 +2016,8 @@ static hipError_t launch_persist(const DGraph &g, const DReads &r, const i32 *or
 
 template <int NW>
 static hipError_t launch_serve(const SrvArgs &a, const PersistCfg &cfg, hipStream_t st) {
-	hipLaunchKernelGGL(gwf_serve_kernel<NW>, dim3(cfg.wgs), dim3(WAVE * NW), cfg.lds_bytes, st, a);
+	auto k = a.g.mixed ? gwf_serve_kernel<NW, true> : gwf_serve_kernel<NW, false>;
+	hipLaunchKernelGGL(k, dim3(cfg.wgs), dim3(WAVE * NW), cfg.lds_bytes, st, a);
 	return hipGetLastError();
 }
 
@@ -1867,9 +2046,9 @@ size_t gwf_max_lds_bytes() { return 160 * 1024 - 4096; }
 template <int NW>
 static int occ(size_t lds_bytes) {
 	int nb = 0;
-	(void)hipFuncSetAttribute((const void *)gwf_persistent_kernel<NW>, hipFuncAttributeMaxDynamicSharedMemorySize,
+	(void)hipFuncSetAttribute((const void *)gwf_persistent_kernel<NW, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
 	                          (int)gwf_max_lds_bytes());
-	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, gwf_persistent_kernel<NW>, WAVE * NW, lds_bytes) != hipSuccess)
+	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, gwf_persistent_kernel<NW, false>, WAVE * NW, lds_bytes) != hipSuccess)
 		return 1;
 	return nb;
 }
@@ -1899,9 +2078,9 @@ int gwf_persist_occupancy(int nw, size_t lds_bytes) {
 template <int NW>
 static int occ_serve(size_t lds_bytes) {
 	int nb = 0;
-	(void)hipFuncSetAttribute((const void *)gwf_serve_kernel<NW>, hipFuncAttributeMaxDynamicSharedMemorySize,
+	(void)hipFuncSetAttribute((const void *)gwf_serve_kernel<NW, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
 	                          (int)gwf_max_lds_bytes());
-	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, gwf_serve_kernel<NW>, WAVE * NW, lds_bytes) != hipSuccess)
+	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, gwf_serve_kernel<NW, false>, WAVE * NW, lds_bytes) != hipSuccess)
 		return 1;
 	return nb;
 }

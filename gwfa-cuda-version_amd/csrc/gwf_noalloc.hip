@@ -111,10 +111,10 @@ extern "C" int gwf_ed_run_noalloc(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t v1
 		i32 end = pos, maxql = 0;
 		while (end < n) {
 			const i32 q = std::max(maxql, qlh[end]);
-			if (end > pos && gwf_ws_bytes(gwf_caps_for(q, 1)) * (size_t)(end - pos + 1) > ws_bytes) break;
+			if (end > pos && gwf_ws_bytes(gwf_caps_for(q, 1, dg->g.mixed != 0)) * (size_t)(end - pos + 1) > ws_bytes) break;
 			maxql = q, ++end;
 		}
-		const Caps c = gwf_caps_for(maxql, 1);
+		const Caps c = gwf_caps_for(maxql, 1, dg->g.mixed != 0);
 		if (gwf_ws_bytes(c) > ws_bytes) {
 			gwf_set_error("gwf_ed_run_noalloc: read %d (length %d) needs a %zu-byte workspace; the scratch holds %zu",
 			              pos, qlh[pos], gwf_ws_bytes(c), ws_bytes);
@@ -131,7 +131,7 @@ extern "C" int gwf_ed_run_noalloc(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t v1
 	while (e == hipSuccess && h[0] > 0) {
 		st.n_retry += h[0];
 		scale *= 4;
-		const Caps c = gwf_caps_for(h[1], scale);
+		const Caps c = gwf_caps_for(h[1], scale, dg->g.mixed != 0);
 		const size_t wb = gwf_ws_bytes(c);
 		if (wb > ws_bytes || (c.x >= (1 << 25) && scale > (1 << 12))) {
 			gwf_set_error("gwf_ed_run_noalloc: %d reads need workspaces of %zu bytes; the scratch holds %zu", h[0], wb,

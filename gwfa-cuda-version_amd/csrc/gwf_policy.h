@@ -11,8 +11,10 @@
 constexpr size_t kSeqPad = 2048;
 
 int64_t gwf_env_int(const char *name, int64_t dflt);
-// HBM workspace capacities of a read of length ql, times `scale` after overflows.
-Caps gwf_caps_for(i32 ql, int scale);
+// HBM workspace capacities of a read of length ql, times `scale` after
+// overflows; `mixed`: the graph has vertices with in-arcs of different
+// overlaps (the workspace then holds the read's visited set too).
+Caps gwf_caps_for(i32 ql, int scale, bool mixed);
 // LDS capacities for a wavefront of up to ~m states.
 Caps gwf_lds_caps(i32 m);
 // Set this thread's gwf_last_error() text.

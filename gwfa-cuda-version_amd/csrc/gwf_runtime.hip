@@ -177,7 +177,9 @@ int64_t env_int(const char *name, int64_t dflt) {
 // HBM workspace of a read of length ql: the reference's per-read sizing
 // (main.cu:498-504) plus headroom, times `scale` after overflows.
 // GWF_DEBUG_CAP_A overrides the base (tests force the re-run path with it).
-Caps caps_for(i32 ql, int scale) {
+// mixed: the graph has vertices whose in-arcs carry different overlaps (the
+// workspace then holds the read's visited set too).
+Caps caps_for(i32 ql, int scale, bool mixed) {
 	const int64_t dbg = env_int("GWF_DEBUG_CAP_A", 0);
 	int64_t a = dbg > 0 ? dbg : (int64_t)(1.6 * ql) + 256;
 	a *= scale;
@@ -188,6 +190,7 @@ Caps caps_for(i32 ql, int scale) {
 	c.f = (i32)a;
 	c.t = (i32)a;
 	c.h = (i32)a;
+	c.v = gwf_vis_slots((i32)a, mixed);
 	return c;
 }
 
@@ -199,12 +202,13 @@ Caps lds_caps(i32 m) {
 	c.f = std::max(64, m / 4);
 	c.t = std::max(64, m / 2);
 	c.h = std::max(64, m / 4);
+	c.v = 0;  // the per-read visited set, when there is one, stays in HBM
 	return c;
 }
 
 // Launch parameters for waves-per-read nw, LDS wavefront capacity m and
 // sort variant mw (GWF_PERSIST_NW / _M / _WGS override, for tests).
-PersistCfg make_cfg(const RunCtx &cx, i32 nw, i32 m, i32 mw, i32 max_ql, bool env_override = true) {
+PersistCfg make_cfg(const RunCtx &cx, i32 nw, i32 m, i32 mw, i32 max_ql, bool mixed, bool env_override = true) {
 	PersistCfg c;
 	if (env_override) {
 		nw = (i32)env_int("GWF_PERSIST_NW", nw);
@@ -214,7 +218,7 @@ PersistCfg make_cfg(const RunCtx &cx, i32 nw, i32 m, i32 mw, i32 max_ql, bool en
 	c.mw = mw;
 	c.lds = lds_caps(m);
 	c.lds_bytes = gwf_lds_bytes(c.lds);
-	c.hbm = caps_for(max_ql, 1);
+	c.hbm = caps_for(max_ql, 1, mixed);
 	const int per_cu = gwf_persist_occupancy(nw, c.lds_bytes);
 	// LDS left over at this occupancy becomes staging room for spilled reads.
 	// The occupancy query accepts sizes at which MI355X keeps one workgroup
@@ -324,7 +328,7 @@ void cache_drop(const gwf_graph_t *g) {
 }  // namespace
 
 int64_t gwf_env_int(const char *name, int64_t dflt) { return env_int(name, dflt); }
-Caps gwf_caps_for(i32 ql, int scale) { return caps_for(ql, scale); }
+Caps gwf_caps_for(i32 ql, int scale, bool mixed) { return caps_for(ql, scale, mixed); }
 Caps gwf_lds_caps(i32 m) { return lds_caps(m); }
 void gwf_set_error(const char *fmt, ...) {
 	char buf[512];
@@ -374,6 +378,7 @@ int gwf_dgraph_upload(const gwf_graph_t *g, gwf_dgraph_t **out) {
 	const u32 nv = g->n_vtx, na = g->n_arc;
 	std::vector<uint4> vinfo(nv);
 	std::vector<i32> in_ol(nv, -1);
+	bool mixed = false;
 	uint64_t total = 0;
 	for (u32 v = 0; v < nv; ++v) {
 		if (g->len[v] == 0 || g->len[v] > (u32)GWF_MAX_VLEN) {
@@ -399,15 +404,9 @@ int gwf_dgraph_upload(const gwf_graph_t *g, gwf_dgraph_t **out) {
 		// The visited set keys a vertex entry by (target, query position), as
 		// gwf-ed.cu:1196 does.  With one overlap per target, every entry under
 		// a key is the same state; with several, the reference keeps the entry
-		// its FIFO closure reaches first, an order the level-parallel closure
-		// does not follow, so such graphs are refused rather than answered
-		// differently.
-		if (in_ol[w] >= 0 && in_ol[w] != o) {
-			set_err("gwf_dgraph_upload: vertex %u has in-arcs with different overlaps (%d, %d); the "
-			        "reference's result then depends on its closure order, which the GPU does not follow",
-			        w, in_ol[w], o);
-			return GWF_ERR_ARG;
-		}
+		// its FIFO closure reaches first, so such graphs are aligned with the
+		// closure in that order and one visited set per read (DGraph::mixed).
+		if (in_ol[w] >= 0 && in_ol[w] != o) mixed = true;
 		in_ol[w] = o;
 		arcs[i] = make_uint4(w, (u32)o | (u32)(uint8_t)g->seq[w][o] << 24, seq_off[w], g->len[w]);
 	}
@@ -443,6 +442,8 @@ int gwf_dgraph_upload(const gwf_graph_t *g, gwf_dgraph_t **out) {
 	if (e == hipSuccess) e = hipMemcpy(p, arcs.data(), b_arc, hipMemcpyHostToDevice);
 	dg->g.n_vtx = nv;
 	dg->g.total_len = (u32)total;
+	dg->g.mixed = mixed ? 1u : 0u;
+	dg->g.pad = 0;
 	dg->n_vtx = nv;
 	dg->gen = gwf_next_generation();
 	if (e != hipSuccess) {
@@ -628,7 +629,7 @@ bool run_plain(RunCtx &cx, RunAcc &acc, const gwf_dgraph_t *dg, const DReads &r,
 	std::vector<size_t> wsb(nt);
 	size_t total = 0;
 	for (size_t j = 0; j < nt; ++j) {
-		const Caps c = caps_for(qlh[todo[j]], scale);
+		const Caps c = caps_for(qlh[todo[j]], scale, dg->g.mixed != 0);
 		wa[j] = c.x;
 		wsb[j] = gwf_ws_bytes(c);
 		total += wsb[j];
@@ -721,8 +722,8 @@ struct BinnedPlan {
 
 // The longest read length (at most maxql) whose HBM workspace, times wgs
 // workgroups, fits the budget (at least 1: the workspace of a read of length 1).
-i32 bin_ws_ql(i32 maxql, i32 wgs, size_t budget) {
-	auto fits = [&](i32 q) { return gwf_ws_bytes(caps_for(q, 1)) * (size_t)std::max(wgs, 1) <= budget; };
+i32 bin_ws_ql(i32 maxql, i32 wgs, size_t budget, bool mixed) {
+	auto fits = [&](i32 q) { return gwf_ws_bytes(caps_for(q, 1, mixed)) * (size_t)std::max(wgs, 1) <= budget; };
 	if (fits(maxql)) return maxql;
 	i32 lo = 1, hi = maxql;  // fits(lo) or lo == 1; !fits(hi)
 	while (hi - lo > 1) {
@@ -782,18 +783,19 @@ bool run_binned(RunCtx &cx, RunAcc &acc, const gwf_dgraph_t *dg, const gwf_dread
 	BinnedPlan p;
 	p.n = n;
 	p.v1 = v1;
-	p.probe = make_cfg(cx, 1, 256, 8, std::min(maxql, kProbeLen));
+	const bool mixed = dg->g.mixed != 0;
+	p.probe = make_cfg(cx, 1, 256, 8, std::min(maxql, kProbeLen), mixed);
 	p.bb.n_bins = t.n_bins;
 	size_t ws_sum = 0, ws_max = 0;
 	for (int b = 0; b < t.n_bins; ++b) {
 		p.bb.est_max[b] = t.bin[b].est_max;
-		p.bin[b] = make_cfg(cx, t.bin[b].nw, t.bin[b].lds_states, t.bin[b].merge_runs, maxql);
+		p.bin[b] = make_cfg(cx, t.bin[b].nw, t.bin[b].lds_states, t.bin[b].merge_runs, maxql, mixed);
 		// Every resident workgroup holds one HBM workspace.  Sized for the
 		// longest read they would exceed the budget when a few reads are far
 		// longer than the rest; the workspaces are then sized for the longest
 		// length that fits, and a read whose state outgrows them reports an
 		// overflow and is re-run by the budgeted plain launches below.
-		p.bin[b].hbm = caps_for(bin_ws_ql(maxql, p.bin[b].wgs, budget), 1);
+		p.bin[b].hbm = caps_for(bin_ws_ql(maxql, p.bin[b].wgs, budget, mixed), 1, mixed);
 		const size_t w = gwf_ws_bytes(p.bin[b].hbm) * (size_t)p.bin[b].wgs;
 		ws_sum += w;
 		ws_max = std::max(ws_max, w);
@@ -954,7 +956,7 @@ int gwf_ed_run_timed(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t v1, uint32_t ma
 		if (!again.empty()) {
 			acc.st.n_retry += (int64_t)again.size();
 			scale *= 4;
-			if (caps_for(1, scale).x >= (1 << 25) && scale > (1 << 12)) {
+			if (caps_for(1, scale, false).x >= (1 << 25) && scale > (1 << 12)) {
 				set_err("gwf_ed_run: %zu reads exceed the maximum workspace", again.size());
 				rc = GWF_ERR_CAPACITY;
 			}

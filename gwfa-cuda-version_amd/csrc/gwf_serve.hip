@@ -231,7 +231,7 @@ void gwf_serve_s::finish(std::unique_ptr<Req> r, const SrvResult *res, int statu
 // Publish the head of `ready` if a slot, budget and staging space are free.
 bool gwf_serve_s::admit_one(int64_t now) {
 	Req &r = *ready.front();
-	const Caps hc = gwf_caps_for(r.ql, r.scale);
+	const Caps hc = gwf_caps_for(r.ql, r.scale, dg->g.mixed != 0);
 	const u64 q_len = round_up((u64)r.ql, 16) + 16;
 	const u64 ws_off = round_up(round_up((u64)r.ql, 16) + kSeqPad, 256);
 	const u64 d_len = ws_off + gwf_ws_bytes(hc);
@@ -305,7 +305,7 @@ void gwf_serve_s::collect(int64_t now) {
 		inflight.pop_back();
 		if (res.status == GWF_ST_OVERFLOW) {
 			r->scale *= 4;
-			if (gwf_caps_for(1, r->scale).x >= (1 << 25) && r->scale > (1 << 12)) {
+			if (gwf_caps_for(1, r->scale, false).x >= (1 << 25) && r->scale > (1 << 12)) {
 				finish(std::move(r), &res, GWF_ERR_CAPACITY, f.t_admit, now);
 			} else {
 				{
@@ -540,7 +540,7 @@ int gwf_serve_start(gwf_dgraph_t *dg, const gwf_serve_opts_t *opts, gwf_serve_t 
 		return GWF_ERR_ARG;
 	}
 	c.wgs = o.wgs > 0 ? o.wgs : std::max(1, gwf_serve_occupancy(c.nw, c.lds_bytes)) * std::max(1, n_cu);
-	c.hbm = gwf_caps_for(1, 1);  // unused: each request brings its own workspace capacities
+	c.hbm = gwf_caps_for(1, 1, false);  // unused: each request brings its own workspace capacities
 	c.mw = 0;
 	if (e == hipSuccess) e = gwf_kernels_init();
 	if (e == hipSuccess) e = hipStreamCreate(&s->stream);

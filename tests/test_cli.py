@@ -58,3 +58,32 @@ def test_cli_eds(gpu, kernel):
     eds = [int(x) for x in lines[2:1002]]
     assert eds == read_ints(os.path.join(GOLDEN, "reads_1000.ed"))
     assert lines[1002] == "" and lines[1003].startswith("Tempo GPU streams: ") and lines[1003].endswith(" secondi")
+
+
+@pytest.mark.gpu
+def test_cli_gfa_with_varying_overlaps(gpu, orc, tmp_path):
+    """A GFA whose links carry different CIGAR overlaps into one segment runs
+    through gwf-test (the reference aligns any GFA, gwf-ed.cu:1186-1212) with
+    the oracle's EDs."""
+    print("case: gwf-test CLI, GFA with varying CIGAR overlaps", flush=True)
+    gfa = tmp_path / "ovl.gfa"
+    gfa.write_text("H\tVN:Z:1.0\n"
+                   "S\ts1\tACGTACGGTA\nS\ts2\tTTGCAGGA\nS\ts3\tGGATCCATGCA\nS\ts4\tCATTGACC\n"
+                   "L\ts1\t+\ts2\t+\t0M\nL\ts1\t+\ts3\t+\t2M\nL\ts2\t+\ts3\t+\t4M\n"
+                   "L\ts3\t+\ts4\t+\t3M\nL\ts2\t+\ts4\t+\t1M\n")
+    reads = ["ACGTACGGTATTGCAGGATCCATGCATTGACC", "ACGTACGGTATCCATGCATTGACC", "ACGTTCGGTATTGCAGGATGCATTGAC",
+             "ACGTACGGTATTGCAGGACATTGACC", "ACGACGGTACCATGCATGACC", "GGGG"]
+    fa = tmp_path / "r.fa"
+    fa.write_text("".join(f">r{i}\n{q}\n" for i, q in enumerate(reads)))
+    g = gpu.Graph.from_gfa(str(gfa))
+    g.index()
+    ins = {}
+    for _, w, o in g.arcs():
+        ins.setdefault(w, set()).add(o)
+    assert any(len(o) > 1 for o in ins.values())          # a segment entered with two different overlaps
+    want = [orc.ed(g, q) for q in reads]
+    p = subprocess.run([_cli(gpu), str(gfa), str(fa)], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    lines = p.stdout.split("\n")
+    assert lines[1] == f" righe: {len(reads)}"
+    assert [int(x) for x in lines[2:2 + len(reads)]] == want

@@ -224,11 +224,106 @@ def test_gwf_ed_single_call(gpu, orc):
         assert gpu.gwf_ed(g, q) == orc.ed(g, q)
 
 
-def test_mixed_overlaps_rejected(gpu):
-    _say("case: mixed in-overlaps rejected")
-    g = gpu.Graph.from_lists(["ACGT", "CCGT", "GATTACA"], [(0, 2, 0), (1, 2, 2), (0, 1, 0)])
-    with pytest.raises(gpu.GwfError):
-        gpu.DeviceGraph(g)
+def _mixed_graph(gw, rng, nv, dag):
+    """A random graph whose arcs carry their own overlaps, so that vertices get
+    in-arcs with different overlaps."""
+    seqs = ["".join(rng.choice("ACGT"[:rng.randint(2, 4)]) for _ in range(rng.randint(1, 3 if rng.random() < 0.4 else 20)))
+            for _ in range(nv)]
+    arcs = set()
+    for v in range(nv):
+        for _ in range(rng.randint(0, 3)):
+            if dag:
+                if v + 1 >= nv:
+                    break
+                w = rng.randint(v + 1, nv - 1)
+            else:
+                w = rng.randrange(nv)
+            arcs.add((v, w, rng.randrange(len(seqs[w]))))
+    return gw.Graph.from_lists(seqs, sorted(arcs)), seqs
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("seed,dag", [(41, True), (42, False)])
+def test_mixed_in_overlaps_vs_oracle(gpu, orc, seed, dag, kernel):
+    """Vertices with in-arcs of different overlaps: the reference's answer
+    depends on the order in which its FIFO closure claims visited keys; the
+    GPU follows that order (and one visited set per read), so every ED equals
+    the oracle's, with v1 < 0 and v1 >= 0."""
+    _say(f"case: mixed in-overlaps, random graphs seed={seed} dag={dag} kernel {kernel}")
+    rng = random.Random(seed)
+    n_mixed = 0
+    for it in range(20):
+        g, seqs = _mixed_graph(gpu, rng, rng.randint(2, 30), dag)
+        arcs_of = {}
+        for v, w, o in g.arcs():
+            arcs_of.setdefault(v, []).append((w, o))
+        n_mixed += len({(w, o) for _, w, o in g.arcs()}) > len({w for _, w, _ in g.arcs()})
+        reads = [_sample(rng, seqs, arcs_of, rng.randint(0, 200), rng.choice([0.0, 0.05, 0.2])) for _ in range(15)]
+        reads += ["".join(rng.choice("ACGT") for _ in range(rng.randint(1, 60)))]
+        r = gpu.Reads.from_list(reads)
+        g.index()
+        for v1 in (-1, rng.randrange(g.n_vtx)):
+            want, _, _ = orc.batch(g, r, v1=v1)
+            dg, dr = gpu.DeviceGraph(g), gpu.DeviceReads(r)
+            try:
+                got, _ = gpu.ed_run(dg, dr, kernel=kernel, v1=v1)
+            finally:
+                dr.free()
+                dg.free()
+            assert np.array_equal(got, want), f"iteration {it} v1={v1}: gpu {got.tolist()} oracle {want.tolist()}"
+    assert n_mixed >= 10
+
+
+def test_mixed_in_overlaps_order_dependent_case(gpu, orc):
+    """The graph of test_oracle's order-dependence case, in both labellings:
+    the reference answers 12 and 11, and so must the GPU."""
+    _say("case: mixed in-overlaps, the order-dependent case in two labellings")
+    seqs = ["CAA", "ACC", "GTGG"]
+    arcs = [(0, 1, 0), (1, 1, 2), (1, 2, 3), (1, 1, 1), (2, 0, 2), (2, 2, 1), (2, 1, 2)]
+    q = "GCGCTCGGAAGTATTATT"
+    perm = [0, 2, 1]
+    inv = {old: new for new, old in enumerate(perm)}
+    for s_, a_ in ((seqs, arcs), ([seqs[o] for o in perm], [(inv[a], inv[b], o) for a, b, o in arcs])):
+        g = gpu.Graph.from_lists(s_, a_)
+        g.index()
+        want = orc.ed(g, q)
+        for kernel in KERNELS:
+            dg, dr = gpu.DeviceGraph(g), gpu.DeviceReads(gpu.Reads.from_list([q]))
+            try:
+                got, _ = gpu.ed_run(dg, dr, kernel=kernel)
+            finally:
+                dr.free()
+                dg.free()
+            assert got.tolist() == [want]
+        assert gpu.gwf_ed(g, q) == want
+
+
+@pytest.mark.parametrize("env", [{"GWF_PERSIST_M": "64"}, {"GWF_DEBUG_CAP_A": "48"}])
+def test_mixed_in_overlaps_spill_and_rerun(gpu, orc, monkeypatch, env):
+    """Mixed in-overlaps through an LDS-to-HBM spill (the spilled step repeats
+    its visited-set claims with the same stamps) and through workspace
+    overflow re-runs (a full visited set re-runs the read with a larger one)."""
+    _say(f"case: mixed in-overlaps, {env}")
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    rng = random.Random(43)
+    for it in range(8):
+        g, seqs = _mixed_graph(gpu, rng, rng.randint(10, 40), it % 2 == 0)
+        arcs_of = {}
+        for v, w, o in g.arcs():
+            arcs_of.setdefault(v, []).append((w, o))
+        reads = [_sample(rng, seqs, arcs_of, rng.randint(50, 300), 0.15) for _ in range(12)]
+        r = gpu.Reads.from_list(reads)
+        g.index()
+        want, _, _ = orc.batch(g, r)
+        dg, dr = gpu.DeviceGraph(g), gpu.DeviceReads(r)
+        try:
+            for kernel in KERNELS:
+                got, st = gpu.ed_run(dg, dr, kernel=kernel)
+                assert np.array_equal(got, want), f"iteration {it} kernel {kernel}"
+        finally:
+            dr.free()
+            dg.free()
 
 
 @pytest.mark.parametrize("kernel", KERNELS)
@@ -275,3 +370,61 @@ def test_end_vertex_v1_config_graph(gpu, orc):
     finally:
         dr.free()
         dg.free()
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
+def test_end_vertex_v1_exact_domain_vs_dp(gpu, orc, kernel):
+    """Where v1 has no out-arc, or out-arcs that do not all start with one base,
+    the reference algorithm's end-at-v1 result is the exact distance (pinned by
+    tests/test_oracle.py), so the GPU must give the end-at-v1 DP itself."""
+    _say(f"case: end vertex v1 in the exact domain, GPU vs the DP, kernel {kernel}")
+    from test_oracle import _v1_domain_case
+    rng = random.Random(37)
+    for it in range(30):
+        g, seqs, v1, _ = _v1_domain_case(gpu, rng, dag=it % 2 == 0)
+        arcs_of = {}
+        for v, w, o in g.arcs():
+            arcs_of.setdefault(v, []).append((w, o))
+        reads = [_sample(rng, seqs, arcs_of, rng.randint(0, 60), rng.choice([0.0, 0.1, 0.3])) for _ in range(10)]
+        want = [orc.dp(g, q, v1=v1) for q in reads]
+        dg, dr = gpu.DeviceGraph(g), gpu.DeviceReads(gpu.Reads.from_list(reads))
+        try:
+            got, _ = gpu.ed_run(dg, dr, kernel=kernel, v1=v1)
+        finally:
+            dr.free()
+            dg.free()
+        assert got.tolist() == want, f"iteration {it} v1={v1}: gpu {got.tolist()} dp {want}"
+
+
+def test_mixed_in_overlaps_order_sensitive_cases(gpu, orc):
+    """Cases where the reference's answer changes when the vertices are
+    relabelled (start kept at 0), i.e. where the order in which its closure
+    claims visited keys decides the ED: the GPU must follow the order in both
+    labellings."""
+    rng = random.Random(44)
+    found, tries = 0, 0
+    while found < 25 and tries < 40000:
+        tries += 1
+        g, seqs = _mixed_graph(gpu, rng, rng.randint(3, 9), rng.random() < 0.5)
+        arcs = g.arcs()
+        g.index()
+        q = "".join(rng.choice("ACGT") for _ in range(rng.randint(5, 25)))
+        perm = [0] + rng.sample(range(1, len(seqs)), len(seqs) - 1)
+        inv = {old: new for new, old in enumerate(perm)}
+        g2 = gpu.Graph.from_lists([seqs[o] for o in perm], [(inv[a], inv[b], o) for a, b, o in arcs])
+        g2.index()
+        a, b = orc.ed(g, q), orc.ed(g2, q)
+        if a == b:
+            continue
+        found += 1
+        _say(f"case: order-sensitive mixed-overlap graph {found} (try {tries}): oracle {a} / {b}")
+        for gg, want in ((g, a), (g2, b)):
+            for kernel in KERNELS:
+                dg, dr = gpu.DeviceGraph(gg), gpu.DeviceReads(gpu.Reads.from_list([q]))
+                try:
+                    got, _ = gpu.ed_run(dg, dr, kernel=kernel)
+                finally:
+                    dr.free()
+                    dg.free()
+                assert got.tolist() == [want], (seqs, arcs, q, kernel)
+    assert found == 25

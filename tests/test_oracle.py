@@ -226,13 +226,14 @@ def test_v1_skip_insertion_case(gw, orc):
 
 
 def test_mixed_in_overlaps_make_the_result_order_dependent(gw, orc):
-    """Why the GPU path refuses a vertex whose in-arcs carry different
-    overlaps: the visited set keys a vertex entry by (target, query position)
-    (gwf-ed.cu:1196), so two arcs entering one vertex at one query position
-    with different overlaps give two different states under one key, and the
-    reference keeps whichever its FIFO closure reaches first.  Relabelling
-    the vertices (start kept at 0) then changes its answer; with one overlap
-    per target it never does (test_invariants)."""
+    """Why the GPU follows the reference's closure order on a graph where one
+    vertex has in-arcs with different overlaps: the visited set keys a vertex
+    entry by (target, query position) (gwf-ed.cu:1196), so two arcs entering
+    one vertex at one query position with different overlaps give two
+    different states under one key, and the reference keeps whichever its FIFO
+    closure reaches first.  Relabelling the vertices (start kept at 0) then
+    changes its answer; with one overlap per target it never does
+    (test_invariants)."""
     seqs = ["CAA", "ACC", "GTGG"]
     arcs = [(0, 1, 0), (1, 1, 2), (1, 2, 3), (1, 1, 1), (2, 0, 2), (2, 2, 1), (2, 1, 2)]
     q = "GCGCTCGGAAGTATTATT"
@@ -243,3 +244,117 @@ def test_mixed_in_overlaps_make_the_result_order_dependent(gw, orc):
     g2 = gw.Graph.from_lists([seqs[o] for o in perm], [(inv[a], inv[b], o) for a, b, o in arcs])
     g2.index()
     assert (orc.ed(g, q), orc.ed(g2, q), orc.dp(g, q)) == (12, 11, 11)
+
+
+# ---- v1 >= 0: the domain where the reference algorithm's end-at-v1 result is
+# exact.  The vertex-end insertion is skipped when every out-arc matches the
+# next query base (gwf-ed.cu:1225).  At any vertex other than v1 that only
+# removes paths dominated by the matching entries, but at v1 it removes the
+# path that inserts the rest of the query after v1's last base.  So whenever
+# v1 has no out-arc, or its out-arcs do not all start with the same base, the
+# skip cannot fire at v1 and the result must be the exact end-at-v1 DP.
+
+def _v1_domain_case(gw, rng, dag):
+    while True:
+        g, seqs = rand_graph(gw, rng, rng.randint(1, 12), dag)
+        outs = {}
+        for a, b, o in g.arcs():
+            outs.setdefault(a, []).append(seqs[b][o])
+        ok = [v for v in range(g.n_vtx) if v not in outs or len(set(outs[v])) >= 2]
+        if ok:
+            return g, seqs, rng.choice(ok), rand_read(rng, rng.randint(0, 25))
+
+
+@pytest.mark.parametrize("dag", [True, False])
+def test_v1_exact_where_the_skip_cannot_fire_at_v1(gw, orc, dag):
+    rng = random.Random(23 + dag)
+    reach = 0
+    for it in range(2500):
+        g, seqs, v1, q = _v1_domain_case(gw, rng, dag)
+        want = orc.dp(g, q, v1=v1)
+        assert orc.ed(g, q, v1=v1) == want, (it, seqs, g.arcs(), q, v1)
+        reach += want >= 0
+    assert reach > 800        # a good share of the cases have an end
+
+
+def levenshtein(a, b):
+    prev = list(range(len(b) + 1))
+    for i, ca in enumerate(a, 1):
+        cur = [i] + [0] * len(b)
+        for j, cb in enumerate(b, 1):
+            cur[j] = min(prev[j] + 1, cur[j - 1] + 1, prev[j - 1] + (ca != cb))
+        prev = cur
+    return prev[-1]
+
+
+def test_v1_closed_forms(gw, orc):
+    rng = random.Random(29)
+    # a read spelling a path exactly, ending at the last base of its last vertex -> 0
+    for _ in range(200):
+        g, seqs = rand_graph(gw, rng, rng.randint(2, 10), dag=True, ovl=rng.random() < 0.5)
+        outs = {}
+        for a, b, o in g.arcs():
+            outs.setdefault(a, []).append((b, o))
+        v, q = 0, seqs[0]
+        for _ in range(rng.randint(0, 6)):
+            if v not in outs:
+                break
+            v, o = rng.choice(outs[v])
+            q += seqs[v][o:]
+        assert orc.ed(g, q, v1=v) == 0 == orc.dp(g, q, v1=v)
+    # a chain ending at its last vertex (a sink): the global edit distance to the
+    # whole chain's sequence; at an earlier vertex never below that vertex's prefix
+    for _ in range(60):
+        seqs = [rand_read(rng, rng.randint(1, 8)) for _ in range(rng.randint(1, 6))]
+        g = gw.Graph.from_lists(seqs, [(i, i + 1) for i in range(len(seqs) - 1)])
+        g.index()
+        q = rand_read(rng, rng.randint(0, 30))
+        last = len(seqs) - 1
+        assert orc.ed(g, q, v1=last) == levenshtein(q, "".join(seqs))
+        j = rng.randrange(len(seqs))
+        assert orc.ed(g, q, v1=j) >= levenshtein(q, "".join(seqs[:j + 1]))
+
+
+# Plausible mistakes in the oracle's end-vertex branches (gwf_oracle.c, the
+# restatement of gwf-ed.cu:1231-1254), each planted in a copy compiled here;
+# the exact-domain check above must reject every one.
+V1_PLANTS = [
+    ("no deletion after a query end mid-vertex (gwf-ed.cu:1237-1241)",
+     "\t\t\tor_B_push(b, v, d - 1, k + 1, x0 + 1, ooo);\n\t\t} else if ((int32_t)v != v1) {",
+     "\t\t\t;\n\t\t} else if ((int32_t)v != v1) {"),
+    ("that deletion on the wrong diagonal",
+     "\t\t\tor_B_push(b, v, d - 1, k + 1, x0 + 1, ooo);\n\t\t} else if ((int32_t)v != v1) {",
+     "\t\t\tor_B_push(b, v, d, k + 1, x0 + 1, ooo);\n\t\t} else if ((int32_t)v != v1) {"),
+    ("no deletion into the next vertices at a vertex end (gwf-ed.cu:1244-1254)",
+     "\t\t\t\tor_B_push(b, w, ii - ol, ol, x0 + 1, 1);\n\t\t\t}\n\t\t}\n\t}",
+     "\t\t\t\t(void)w, (void)ol;\n\t\t\t}\n\t\t}\n\t}"),
+    ("those deletions through the first out-arc only",
+     "nv = (int32_t)g->aux[v], j;\n\t\t\tfor (j = 0; j < nv; ++j) {",
+     "nv = (int32_t)g->aux[v], j;\n\t\t\tfor (j = 0; j < (nv < 1 ? nv : 1); ++j) {"),
+    ("an end anywhere in v1 instead of at its last base (gwf-ed.cu:1231)",
+     "((int32_t)v == v1 && k + 1 == vl)", "((int32_t)v == v1)"),
+]
+
+
+def test_v1_pin_rejects_planted_mistakes(gw, orc, tmp_path):
+    import ctypes as C
+    import subprocess
+    src = open(os.path.join(os.path.dirname(orc.__file__), "gwf_oracle.c")).read()
+    rng = random.Random(31)
+    cases = [_v1_domain_case(gw, rng, dag) for dag in (True, False) for _ in range(700)]
+    for i, (what, old, new) in enumerate(V1_PLANTS):
+        assert src.count(old) == 1, what
+        path = tmp_path / f"plant{i}.c"
+        path.write_text(src.replace(old, new))
+        so = tmp_path / f"plant{i}.so"
+        subprocess.run(["gcc", "-O1", "-fPIC", "-shared", "-w", f"-I{os.path.dirname(orc.__file__)}", "-o", str(so),
+                        str(path), "-lpthread"], check=True)
+        lib = C.CDLL(str(so))
+        lib.oracle_gwf_ed.argtypes = [C.c_void_p, C.c_int32, C.c_char_p, C.c_int32, C.c_int32, C.c_uint32,
+                                      C.c_void_p]
+        lib.oracle_gwf_ed.restype = C.c_int32
+        wrong = 0
+        for g, _, v1, q in cases:
+            qb = q.encode()
+            wrong += lib.oracle_gwf_ed(C.cast(g.ptr, C.c_void_p), len(qb), qb, 0, v1, 0, None) != orc.dp(g, q, v1=v1)
+        assert wrong > 0, f"planted mistake not caught: {what}"
