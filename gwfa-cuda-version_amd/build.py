@@ -48,16 +48,43 @@ def build_host():
 
 
 HIP_SOURCES = ["gwf_kernels.hip", "gwf_runtime.hip", "gwf_serve.hip", "gwf_noalloc.hip"]
+# gwf_kernels.hip is compiled once per state-key layout (the split of a
+# state's 64-bit key between vertex, diagonal and offset; gwf_dev.h); the
+# layout-0 object also holds the layout-independent code.
+KEY_LAYOUTS = 4
+
+
+def _compile_jobs():
+    jobs = []
+    for name in HIP_SOURCES:
+        src = os.path.join(CSRC, name)
+        if name == "gwf_kernels.hip":
+            for lay in range(KEY_LAYOUTS):
+                jobs.append((src, os.path.join(LIB, f"{name}.l{lay}.o"), [f"-DGWF_KEY_LAYOUT={lay}"]))
+        else:
+            jobs.append((src, os.path.join(LIB, name + ".o"), []))
+    return jobs
 
 
 def build_hip():
     os.makedirs(LIB, exist_ok=True)
-    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    objs = []
-    for s in srcs:
-        o = os.path.join(LIB, os.path.basename(s) + ".o")
-        _run([HIPCC, *HIP_FLAGS, f"-I{INC}", "-c", s, "-o", o])
-        objs.append(o)
+    jobs = _compile_jobs()
+    width = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
+    running, objs, failed = [], [], None
+    for src, obj, extra in jobs:
+        cmd = [HIPCC, *HIP_FLAGS, *extra, f"-I{INC}", "-c", src, "-o", obj]
+        print("+", " ".join(cmd), flush=True)
+        running.append((subprocess.Popen(cmd), cmd))
+        objs.append(obj)
+        while len(running) >= width:
+            p, c = running.pop(0)
+            if p.wait() != 0 and failed is None:
+                failed = c
+    for p, c in running:
+        if p.wait() != 0 and failed is None:
+            failed = c
+    if failed is not None:
+        raise subprocess.CalledProcessError(1, failed)
     out = os.path.join(LIB, "libgwfa_hip.so")
     _atomic_link([HIPCC, "-shared", f"--offload-arch={ARCH}", *objs, "-lpthread"], out)
     for o in objs:

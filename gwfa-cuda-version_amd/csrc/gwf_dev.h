@@ -7,19 +7,45 @@ typedef uint64_t u64;
 typedef uint32_t u32;
 typedef int32_t i32;
 
-// A wavefront state (vertex v, diagonal d, offset k) is packed into one u64:
-//   bits 63..40  v                     (n_vtx  < 2^24)
-//   bits 39..20  d + GWF_D_BIAS        (|d|    < 2^19)
-//   bits 19..0   GWF_KINV_MAX - k      (-1 <= k <= GWF_KINV_MAX)
-// so ascending keys sort by vertex, then diagonal, then k descending; the
-// first key of every (v, d) group is the furthest-reaching state.  The top 44
-// bits ("diag id") identify (v, d); consecutive diagonals of one vertex have
-// consecutive diag ids, which is what the forbidden-interval history stores.
-#define GWF_D_BIAS   (1 << 19)
-#define GWF_KINV_MAX ((1 << 20) - 2)
-#define GWF_MAX_VTX  (1u << 24)
-#define GWF_MAX_VLEN ((1 << 19) - 1)
-#define GWF_MAX_QLEN ((1 << 19) - 2)
+// A wavefront state (vertex v, diagonal d, offset k) is packed into one u64
+// of three fields, from the top: v (bv bits), d + 2^(bd-1) (bd bits) and
+// 2^bk - 2 - k (bk bits, -1 <= k <= 2^bk - 2), so ascending keys sort by
+// vertex, then diagonal, then k descending; the first key of every (v, d)
+// group is the furthest-reaching state.  The top bv + bd bits ("diag id")
+// identify (v, d); consecutive diagonals of one vertex have consecutive diag
+// ids, which is what the forbidden-interval history stores.  The kernels are
+// built for GWF_N_LAYOUTS splits; a run uses the first that fits its graph and
+// reads (gwf_pick_layout): v must stay below 2^bv - 1 (all ones is the empty
+// key's), -2^(bd-1) < d < 2^(bd-1) and -1 <= k <= 2^bk - 2; d = i - k runs
+// from -(vertex length) to read length + 1.
+struct GwfKeyLayout {
+	i32 bv, bd, bk;
+};
+#define GWF_N_LAYOUTS 4
+__host__ __device__ constexpr GwfKeyLayout gwf_key_layout(int l) {
+	return l == 1 ? GwfKeyLayout{28, 18, 18}    // large graphs
+	     : l == 2 ? GwfKeyLayout{20, 22, 22}    // long reads and segments
+	     : l == 3 ? GwfKeyLayout{32, 16, 16}    // any 32-bit vertex id
+	              : GwfKeyLayout{24, 20, 20};
+}
+// Largest vertex count, vertex length and read length layout l represents.
+inline uint64_t gwf_layout_max_vtx(int l) { return (1ull << gwf_key_layout(l).bv) - 2; }
+inline u32 gwf_layout_max_vlen(int l) {
+	const GwfKeyLayout k = gwf_key_layout(l);
+	const u32 a = (1u << (k.bd - 1)) - 1, b = (1u << k.bk) - 2;
+	return a < b ? a : b;
+}
+inline i32 gwf_layout_max_ql(int l) { return (1 << (gwf_key_layout(l).bd - 1)) - 2; }
+// The first layout that fits, or -1.
+inline int gwf_pick_layout(uint64_t n_vtx, u32 max_vlen, i32 max_ql) {
+	for (int l = 0; l < GWF_N_LAYOUTS; ++l)
+		if (n_vtx <= gwf_layout_max_vtx(l) && max_vlen <= gwf_layout_max_vlen(l) && max_ql <= gwf_layout_max_ql(l))
+			return l;
+	return -1;
+}
+// Bounds over all layouts (what an upload accepts before a run picks one).
+#define GWF_MAX_VLEN ((1 << 21) - 1)
+#define GWF_MAX_QLEN ((1 << 21) - 2)
 
 // Device graph (packed CSR).  vinfo[v]: x = byte offset of v's sequence in
 // `seq`, y = length, z = first out-arc, w = out-degree.  arc[j] carries what
@@ -32,7 +58,7 @@ struct DGraph {
 	u32 n_vtx;
 	u32 total_len;  // sum of vertex lengths
 	u32 mixed;      // 1: some vertex has in-arcs with different overlaps (per-read visited set)
-	u32 pad;
+	u32 layout;     // key layout of the kernels a launch runs (gwf_key_layout)
 };
 
 struct DReads {

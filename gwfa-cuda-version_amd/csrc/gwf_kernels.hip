@@ -27,20 +27,40 @@
 // with a larger workspace.  All arithmetic is exact integer.
 #include "gwf_dev.h"
 
+// This file is compiled once per key layout (GWF_KEY_LAYOUT = 0 .. GWF_N_LAYOUTS - 1,
+// gwf_key_layout in gwf_dev.h); the layout's kernels live in namespace
+// gwf_layout<n>.  Layout 0's build also holds the code that does not depend
+// on the layout (binning, fills, sizes, occupancy) and the dispatchers that
+// launch the layout a DGraph names.
+#ifndef GWF_KEY_LAYOUT
+#define GWF_KEY_LAYOUT 0
+#endif
+#define GWF_KNS_CAT2(a, b) a##b
+#define GWF_KNS_CAT(a, b) GWF_KNS_CAT2(a, b)
+#define GWF_KNS GWF_KNS_CAT(gwf_layout, GWF_KEY_LAYOUT)
+
 #define WAVE 64
 #define VIS_SLOTS 256      // per-step visited set (w, i+1), LDS
 #define SORT_CHUNK 1024    // plain kernel: run length sorted in LDS
 #define EMPTY_KEY (~0ull)
 
+namespace GWF_KNS {
+
 // ------------------------------------------------------------ key helpers
+// A state (v, d, k) packed as v | d + D_BIAS | KINV_MAX - k in fields of
+// KL.bv, KL.bd and KL.bk bits (see gwf_dev.h).
+constexpr GwfKeyLayout KL = gwf_key_layout(GWF_KEY_LAYOUT);
+constexpr i32 D_BIAS = 1 << (KL.bd - 1);
+constexpr i32 KINV_MAX = (1 << KL.bk) - 2;
+constexpr u32 D_MASK = (1u << KL.bd) - 1, K_MASK = (1u << KL.bk) - 1;
 __device__ __forceinline__ u64 mk(u32 v, i32 d, i32 k) {
-	return (u64)v << 40 | (u64)(u32)(d + GWF_D_BIAS) << 20 | (u64)(u32)(GWF_KINV_MAX - k);
+	return (u64)v << (KL.bd + KL.bk) | (u64)(u32)(d + D_BIAS) << KL.bk | (u64)(u32)(KINV_MAX - k);
 }
-__device__ __forceinline__ u32 key_v(u64 x) { return (u32)(x >> 40); }
-__device__ __forceinline__ i32 key_d(u64 x) { return (i32)((x >> 20) & 0xFFFFFu) - GWF_D_BIAS; }
-__device__ __forceinline__ i32 key_k(u64 x) { return GWF_KINV_MAX - (i32)(x & 0xFFFFFu); }
-__device__ __forceinline__ u64 key_diag(u64 x) { return x >> 20; }
-__device__ __forceinline__ u64 diag_id(u32 v, i32 d) { return (u64)v << 20 | (u64)(u32)(d + GWF_D_BIAS); }
+__device__ __forceinline__ u32 key_v(u64 x) { return (u32)(x >> (KL.bd + KL.bk)); }
+__device__ __forceinline__ i32 key_d(u64 x) { return (i32)((x >> KL.bk) & D_MASK) - D_BIAS; }
+__device__ __forceinline__ i32 key_k(u64 x) { return KINV_MAX - (i32)(x & K_MASK); }
+__device__ __forceinline__ u64 key_diag(u64 x) { return x >> KL.bk; }
+__device__ __forceinline__ u64 diag_id(u32 v, i32 d) { return (u64)v << KL.bd | (u64)(u32)(d + D_BIAS); }
 
 // ------------------------------------------------------------ wave helpers
 __device__ __forceinline__ u32 lane_id() { return __lane_id(); }
@@ -835,13 +855,11 @@ __host__ __device__ inline size_t ws_elems(const Caps &c) {
 	return 2 * (size_t)c.m + c.x + 2 * (size_t)c.f + c.t + 4 * (size_t)c.h + (size_t)(c.x > c.t ? c.x : c.t) + 8 +
 	       2 * (size_t)c.v;
 }
-size_t gwf_ws_bytes(const Caps &c) { return ((ws_elems(c) * 8 + 255) / 256) * 256; }
 // elements between equal plain-kernel workspaces (gwf_ws_bytes / 8)
 __host__ __device__ inline u64 plain_stride(const Caps &c) { return (ws_elems(c) * 8 + 255) / 256 * 32; }
 __host__ __device__ inline size_t lds_elems(const Caps &c) {
 	return 2 * (size_t)c.m + c.x + 2 * (size_t)c.f + c.t + 4 * (size_t)c.h;
 }
-size_t gwf_lds_bytes(const Caps &c) { return 8 * lds_elems(c); }
 
 __device__ inline Bufs carve(u64 *p, const Caps &c, bool with_tmp) {
 	Bufs w;
@@ -1876,6 +1894,114 @@ __global__ __launch_bounds__(WAVE *NW) __attribute__((amdgpu_waves_per_eu(4))) v
 	}
 }
 
+// ------------------------------------------------------------ launchers of this layout
+hipError_t launch_plain(const DGraph &g, const DReads &r, const i32 *read_idx, i32 n, const u64 *ws_off, const i32 *ws_a,
+                        u64 *scratch, i32 *out_ed, i32 *out_status, int64_t *out_counts, hipStream_t st) {
+	if (n <= 0) return hipSuccess;
+	if (g.mixed)
+		hipLaunchKernelGGL(gwf_plain_kernel<true>, dim3(n), dim3(WAVE), 0, st, g, r, read_idx, 0, n, ws_off, ws_a, 0,
+		                   scratch, out_ed, out_status, out_counts);
+	else
+		hipLaunchKernelGGL(gwf_plain_kernel<false>, dim3(n), dim3(WAVE), 0, st, g, r, read_idx, 0, n, ws_off, ws_a, 0,
+		                   scratch, out_ed, out_status, out_counts);
+	return hipGetLastError();
+}
+
+hipError_t launch_plain_uniform(const DGraph &g, const DReads &r, const i32 *read_idx, i32 base, i32 n, i32 a,
+                                u64 *scratch, i32 *out_ed, i32 *out_status, int64_t *out_counts, hipStream_t st) {
+	if (n <= 0) return hipSuccess;
+	if (g.mixed)
+		hipLaunchKernelGGL(gwf_plain_kernel<true>, dim3(n), dim3(WAVE), 0, st, g, r, read_idx, base, n, nullptr, nullptr,
+		                   a, scratch, out_ed, out_status, out_counts);
+	else
+		hipLaunchKernelGGL(gwf_plain_kernel<false>, dim3(n), dim3(WAVE), 0, st, g, r, read_idx, base, n, nullptr, nullptr,
+		                   a, scratch, out_ed, out_status, out_counts);
+	return hipGetLastError();
+}
+
+template <int NW>
+static hipError_t launch_persist(const DGraph &g, const DReads &r, const i32 *order, const i32 *bin_off, i32 bin,
+                                 i32 *heads, const PersistCfg &cfg, u64 *scratch, size_t ws_bytes, i32 *out_ed,
+                                 i32 *out_status, int64_t *out_counts, GwfBeat *beat, hipStream_t st) {
+	auto k = g.mixed ? gwf_persistent_kernel<NW, true> : gwf_persistent_kernel<NW, false>;
+	hipLaunchKernelGGL(k, dim3(cfg.wgs), dim3(WAVE * NW), cfg.lds_bytes, st, g, r, order, bin_off, bin, heads, cfg.lds,
+	                   cfg.hbm, cfg.mw, (i32)(cfg.lds_bytes / 8), scratch, (u64)(ws_bytes / 8), out_ed, out_status,
+	                   out_counts, beat);
+	return hipGetLastError();
+}
+
+hipError_t launch_persistent(const DGraph &g, const DReads &r, const i32 *order, const i32 *bin_off, i32 bin,
+                             i32 *heads, const PersistCfg &cfg, u64 *scratch, size_t ws_bytes, i32 *out_ed,
+                             i32 *out_status, int64_t *out_counts, GwfBeat *beat, hipStream_t st) {
+	switch (cfg.nw) {
+	case 1: return launch_persist<1>(g, r, order, bin_off, bin, heads, cfg, scratch, ws_bytes, out_ed, out_status, out_counts, beat, st);
+	case 2: return launch_persist<2>(g, r, order, bin_off, bin, heads, cfg, scratch, ws_bytes, out_ed, out_status, out_counts, beat, st);
+	case 4: return launch_persist<4>(g, r, order, bin_off, bin, heads, cfg, scratch, ws_bytes, out_ed, out_status, out_counts, beat, st);
+	default: return hipErrorInvalidValue;
+	}
+}
+
+template <int NW>
+static hipError_t launch_serve_nw(const SrvArgs &a, const PersistCfg &cfg, hipStream_t st) {
+	auto k = a.g.mixed ? gwf_serve_kernel<NW, true> : gwf_serve_kernel<NW, false>;
+	hipLaunchKernelGGL(k, dim3(cfg.wgs), dim3(WAVE * NW), cfg.lds_bytes, st, a);
+	return hipGetLastError();
+}
+
+hipError_t launch_serve(const SrvArgs &a, const PersistCfg &cfg, hipStream_t st) {
+	switch (cfg.nw) {
+	case 1: return launch_serve_nw<1>(a, cfg, st);
+	case 2: return launch_serve_nw<2>(a, cfg, st);
+	case 4: return launch_serve_nw<4>(a, cfg, st);
+	default: return hipErrorInvalidValue;
+	}
+}
+
+// Set every dynamic-LDS limit of this layout's kernels (once, outside any graph capture).
+hipError_t kernels_init() {
+	const int mx = (int)gwf_max_lds_bytes();
+	hipError_t e = hipSuccess;
+	const void *ks[] = {(const void *)gwf_persistent_kernel<1, false>, (const void *)gwf_persistent_kernel<2, false>,
+	                    (const void *)gwf_persistent_kernel<4, false>, (const void *)gwf_serve_kernel<1, false>,
+	                    (const void *)gwf_serve_kernel<2, false>, (const void *)gwf_serve_kernel<4, false>,
+	                    (const void *)gwf_persistent_kernel<1, true>, (const void *)gwf_persistent_kernel<2, true>,
+	                    (const void *)gwf_persistent_kernel<4, true>, (const void *)gwf_serve_kernel<1, true>,
+	                    (const void *)gwf_serve_kernel<2, true>, (const void *)gwf_serve_kernel<4, true>};
+	for (const void *k : ks)
+		if (e == hipSuccess) e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
+	return e;
+}
+
+#if GWF_KEY_LAYOUT == 0
+// occupancy queries (the register use of the layouts' kernels is the same)
+template <int NW>
+static int occ(size_t lds_bytes) {
+	int nb = 0;
+	(void)hipFuncSetAttribute((const void *)gwf_persistent_kernel<NW, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+	                          (int)gwf_max_lds_bytes());
+	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, gwf_persistent_kernel<NW, false>, WAVE * NW, lds_bytes) != hipSuccess)
+		return 1;
+	return nb;
+}
+
+template <int NW>
+static int occ_serve(size_t lds_bytes) {
+	int nb = 0;
+	(void)hipFuncSetAttribute((const void *)gwf_serve_kernel<NW, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+	                          (int)gwf_max_lds_bytes());
+	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, gwf_serve_kernel<NW, false>, WAVE * NW, lds_bytes) != hipSuccess)
+		return 1;
+	return nb;
+}
+#endif
+
+}  // namespace GWF_KNS
+
+#if GWF_KEY_LAYOUT == 0
+// ------------------------------------------------------------ shared by every layout
+size_t gwf_ws_bytes(const Caps &c) { return ((gwf_layout0::ws_elems(c) * 8 + 255) / 256) * 256; }
+size_t gwf_lds_bytes(const Caps &c) { return 8 * gwf_layout0::lds_elems(c); }
+
 // ------------------------------------------------------------ device-side binning
 __global__ __launch_bounds__(256) void gwf_bin_zero_kernel(i32 *hist, i32 nh, i32 *heads) {
 	const i32 t = (i32)(blockIdx.x * 256 + threadIdx.x);
@@ -1948,21 +2074,6 @@ hipError_t gwf_launch_fill(i32 *p, i32 n, i32 v, hipStream_t st) {
 	return hipGetLastError();
 }
 
-// Set every dynamic-LDS limit once, outside any graph capture.
-hipError_t gwf_kernels_init() {
-	const int mx = (int)gwf_max_lds_bytes();
-	hipError_t e = hipSuccess;
-	const void *ks[] = {(const void *)gwf_persistent_kernel<1, false>, (const void *)gwf_persistent_kernel<2, false>,
-	                    (const void *)gwf_persistent_kernel<4, false>, (const void *)gwf_serve_kernel<1, false>,
-	                    (const void *)gwf_serve_kernel<2, false>, (const void *)gwf_serve_kernel<4, false>,
-	                    (const void *)gwf_persistent_kernel<1, true>, (const void *)gwf_persistent_kernel<2, true>,
-	                    (const void *)gwf_persistent_kernel<4, true>, (const void *)gwf_serve_kernel<1, true>,
-	                    (const void *)gwf_serve_kernel<2, true>, (const void *)gwf_serve_kernel<4, true>};
-	for (const void *k : ks)
-		if (e == hipSuccess) e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, mx);
-	return e;
-}
-
 hipError_t gwf_launch_binning(const i32 *probe_ed, const i32 *probe_status, const i32 *ql, i32 n, const BinBounds &bb,
                               i32 *key, i32 *hist, i32 *order, i32 *bin_off, i32 *heads, hipStream_t st) {
 	if (bb.n_bins < 1 || bb.n_bins > GWF_MAX_BINS || n < 0) return hipErrorInvalidValue;
@@ -1977,81 +2088,63 @@ hipError_t gwf_launch_binning(const i32 *probe_ed, const i32 *probe_status, cons
 	return hipGetLastError();
 }
 
+#define GWF_LAYOUT_DECLS(NS)                                                                                     \
+	namespace NS {                                                                                                   \
+	hipError_t launch_plain(const DGraph &, const DReads &, const i32 *, i32, const u64 *, const i32 *, u64 *, i32 *, \
+	                        i32 *, int64_t *, hipStream_t);                                                           \
+	hipError_t launch_plain_uniform(const DGraph &, const DReads &, const i32 *, i32, i32, i32, u64 *, i32 *, i32 *,  \
+	                                int64_t *, hipStream_t);                                                          \
+	hipError_t launch_persistent(const DGraph &, const DReads &, const i32 *, const i32 *, i32, i32 *,               \
+	                             const PersistCfg &, u64 *, size_t, i32 *, i32 *, int64_t *, GwfBeat *, hipStream_t); \
+	hipError_t launch_serve(const SrvArgs &, const PersistCfg &, hipStream_t);                                       \
+	hipError_t kernels_init();                                                                                       \
+	}
+GWF_LAYOUT_DECLS(gwf_layout1)
+GWF_LAYOUT_DECLS(gwf_layout2)
+GWF_LAYOUT_DECLS(gwf_layout3)
+static_assert(GWF_N_LAYOUTS == 4, "one namespace per layout");
+
+// the layout named by g.layout (set by the runtime for the graph and reads of a run)
+#define GWF_BY_LAYOUT(layout, call)                      \
+	switch (layout) {                                    \
+	case 0: return gwf_layout0::call;                    \
+	case 1: return gwf_layout1::call;                    \
+	case 2: return gwf_layout2::call;                    \
+	case 3: return gwf_layout3::call;                    \
+	default: return hipErrorInvalidValue;                \
+	}
+
 hipError_t gwf_launch_plain(const DGraph &g, const DReads &r, const i32 *read_idx, i32 n, const u64 *ws_off,
                             const i32 *ws_a, u64 *scratch, i32 *out_ed, i32 *out_status, int64_t *out_counts,
                             hipStream_t st) {
-	if (n <= 0) return hipSuccess;
-	if (g.mixed)
-		hipLaunchKernelGGL(gwf_plain_kernel<true>, dim3(n), dim3(WAVE), 0, st, g, r, read_idx, 0, n, ws_off, ws_a, 0,
-		                   scratch, out_ed, out_status, out_counts);
-	else
-		hipLaunchKernelGGL(gwf_plain_kernel<false>, dim3(n), dim3(WAVE), 0, st, g, r, read_idx, 0, n, ws_off, ws_a, 0,
-		                   scratch, out_ed, out_status, out_counts);
-	return hipGetLastError();
+	GWF_BY_LAYOUT(g.layout, launch_plain(g, r, read_idx, n, ws_off, ws_a, scratch, out_ed, out_status, out_counts, st))
 }
 
 hipError_t gwf_launch_plain_uniform(const DGraph &g, const DReads &r, const i32 *read_idx, i32 base, i32 n, i32 a,
                                     u64 *scratch, i32 *out_ed, i32 *out_status, int64_t *out_counts, hipStream_t st) {
-	if (n <= 0) return hipSuccess;
-	if (g.mixed)
-		hipLaunchKernelGGL(gwf_plain_kernel<true>, dim3(n), dim3(WAVE), 0, st, g, r, read_idx, base, n, nullptr, nullptr,
-		                   a, scratch, out_ed, out_status, out_counts);
-	else
-		hipLaunchKernelGGL(gwf_plain_kernel<false>, dim3(n), dim3(WAVE), 0, st, g, r, read_idx, base, n, nullptr, nullptr,
-		                   a, scratch, out_ed, out_status, out_counts);
-	return hipGetLastError();
-}
-
-template <int NW>
-static hipError_t launch_persist(const DGraph &g, const DReads &r, const i32 *order, const i32 *bin_off, i32 bin,
-                                 i32 *heads, const PersistCfg &cfg, u64 *scratch, size_t ws_bytes, i32 *out_ed,
-                                 i32 *out_status, int64_t *out_counts, GwfBeat *beat, hipStream_t st) {
-	auto k = g.mixed ? gwf_persistent_kernel<NW, true> : gwf_persistent_kernel<NW, false>;
-	hipLaunchKernelGGL(k, dim3(cfg.wgs), dim3(WAVE * NW), cfg.lds_bytes, st, g, r, order,
-	                   bin_off, bin, heads, cfg.lds, cfg.hbm, cfg.mw, (i32)(cfg.lds_bytes / 8), scratch, (u64)(ws_bytes / 8),
-	                   out_ed, out_status,
-	                   out_counts, beat);
-	return hipGetLastError();
-}
-
-template <int NW>
-static hipError_t launch_serve(const SrvArgs &a, const PersistCfg &cfg, hipStream_t st) {
-	auto k = a.g.mixed ? gwf_serve_kernel<NW, true> : gwf_serve_kernel<NW, false>;
-	hipLaunchKernelGGL(k, dim3(cfg.wgs), dim3(WAVE * NW), cfg.lds_bytes, st, a);
-	return hipGetLastError();
-}
-
-hipError_t gwf_launch_serve(const SrvArgs &a, const PersistCfg &cfg, hipStream_t st) {
-	switch (cfg.nw) {
-	case 1: return launch_serve<1>(a, cfg, st);
-	case 2: return launch_serve<2>(a, cfg, st);
-	case 4: return launch_serve<4>(a, cfg, st);
-	default: return hipErrorInvalidValue;
-	}
+	GWF_BY_LAYOUT(g.layout, launch_plain_uniform(g, r, read_idx, base, n, a, scratch, out_ed, out_status, out_counts, st))
 }
 
 hipError_t gwf_launch_persistent(const DGraph &g, const DReads &r, const i32 *order, const i32 *bin_off, i32 bin,
                                  i32 *heads, const PersistCfg &cfg, u64 *scratch, size_t ws_bytes, i32 *out_ed,
                                  i32 *out_status, int64_t *out_counts, GwfBeat *beat, hipStream_t st) {
-	switch (cfg.nw) {
-	case 1: return launch_persist<1>(g, r, order, bin_off, bin, heads, cfg, scratch, ws_bytes, out_ed, out_status, out_counts, beat, st);
-	case 2: return launch_persist<2>(g, r, order, bin_off, bin, heads, cfg, scratch, ws_bytes, out_ed, out_status, out_counts, beat, st);
-	case 4: return launch_persist<4>(g, r, order, bin_off, bin, heads, cfg, scratch, ws_bytes, out_ed, out_status, out_counts, beat, st);
-	default: return hipErrorInvalidValue;
-	}
+	GWF_BY_LAYOUT(g.layout, launch_persistent(g, r, order, bin_off, bin, heads, cfg, scratch, ws_bytes, out_ed, out_status,
+	                                          out_counts, beat, st))
+}
+
+hipError_t gwf_launch_serve(const SrvArgs &a, const PersistCfg &cfg, hipStream_t st) {
+	GWF_BY_LAYOUT(a.g.layout, launch_serve(a, cfg, st))
+}
+
+hipError_t gwf_kernels_init() {
+	hipError_t e = gwf_layout0::kernels_init();
+	if (e == hipSuccess) e = gwf_layout1::kernels_init();
+	if (e == hipSuccess) e = gwf_layout2::kernels_init();
+	if (e == hipSuccess) e = gwf_layout3::kernels_init();
+	return e;
 }
 
 size_t gwf_max_lds_bytes() { return 160 * 1024 - 4096; }
-
-template <int NW>
-static int occ(size_t lds_bytes) {
-	int nb = 0;
-	(void)hipFuncSetAttribute((const void *)gwf_persistent_kernel<NW, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-	                          (int)gwf_max_lds_bytes());
-	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, gwf_persistent_kernel<NW, false>, WAVE * NW, lds_bytes) != hipSuccess)
-		return 1;
-	return nb;
-}
 
 // The largest dynamic LDS size (a multiple of 256 bytes, at least lds_bytes)
 // that keeps per_cu workgroups of nw waves resident per CU, leaving `margin`
@@ -2068,28 +2161,19 @@ size_t gwf_persist_grow_lds(int nw, size_t lds_bytes, int per_cu, size_t margin)
 
 int gwf_persist_occupancy(int nw, size_t lds_bytes) {
 	switch (nw) {
-	case 1: return occ<1>(lds_bytes);
-	case 2: return occ<2>(lds_bytes);
-	case 4: return occ<4>(lds_bytes);
+	case 1: return gwf_layout0::occ<1>(lds_bytes);
+	case 2: return gwf_layout0::occ<2>(lds_bytes);
+	case 4: return gwf_layout0::occ<4>(lds_bytes);
 	default: return 1;
 	}
-}
-
-template <int NW>
-static int occ_serve(size_t lds_bytes) {
-	int nb = 0;
-	(void)hipFuncSetAttribute((const void *)gwf_serve_kernel<NW, false>, hipFuncAttributeMaxDynamicSharedMemorySize,
-	                          (int)gwf_max_lds_bytes());
-	if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, gwf_serve_kernel<NW, false>, WAVE * NW, lds_bytes) != hipSuccess)
-		return 1;
-	return nb;
 }
 
 int gwf_serve_occupancy(int nw, size_t lds_bytes) {
 	switch (nw) {
-	case 1: return occ_serve<1>(lds_bytes);
-	case 2: return occ_serve<2>(lds_bytes);
-	case 4: return occ_serve<4>(lds_bytes);
+	case 1: return gwf_layout0::occ_serve<1>(lds_bytes);
+	case 2: return gwf_layout0::occ_serve<2>(lds_bytes);
+	case 4: return gwf_layout0::occ_serve<4>(lds_bytes);
 	default: return 1;
 	}
 }
+#endif  // GWF_KEY_LAYOUT == 0

@@ -74,6 +74,8 @@ extern "C" int gwf_ed_run_noalloc(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t v1
 			return GWF_ERR_ARG;
 		}
 	if (n == 0) { if (stats) *stats = st; return GWF_OK; }
+	DGraph g;
+	if (int rc = gwf_run_graph(dg, dr, "gwf_ed_run_noalloc", &g)) return rc;
 
 	std::lock_guard<std::mutex> lk(g_mu);
 	int dev = 0;
@@ -111,16 +113,16 @@ extern "C" int gwf_ed_run_noalloc(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t v1
 		i32 end = pos, maxql = 0;
 		while (end < n) {
 			const i32 q = std::max(maxql, qlh[end]);
-			if (end > pos && gwf_ws_bytes(gwf_caps_for(q, 1, dg->g.mixed != 0)) * (size_t)(end - pos + 1) > ws_bytes) break;
+			if (end > pos && gwf_ws_bytes(gwf_caps_for(q, 1, g.mixed != 0)) * (size_t)(end - pos + 1) > ws_bytes) break;
 			maxql = q, ++end;
 		}
-		const Caps c = gwf_caps_for(maxql, 1, dg->g.mixed != 0);
+		const Caps c = gwf_caps_for(maxql, 1, g.mixed != 0);
 		if (gwf_ws_bytes(c) > ws_bytes) {
 			gwf_set_error("gwf_ed_run_noalloc: read %d (length %d) needs a %zu-byte workspace; the scratch holds %zu",
 			              pos, qlh[pos], gwf_ws_bytes(c), ws_bytes);
 			return GWF_ERR_CAPACITY;
 		}
-		e = gwf_launch_plain_uniform(dg->g, rv, nullptr, pos, end - pos, c.x, ws, ed, status, nullptr, nullptr);
+		e = gwf_launch_plain_uniform(g, rv, nullptr, pos, end - pos, c.x, ws, ed, status, nullptr, nullptr);
 		st.n_launch += 1;
 		pos = end;
 	}
@@ -131,7 +133,7 @@ extern "C" int gwf_ed_run_noalloc(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t v1
 	while (e == hipSuccess && h[0] > 0) {
 		st.n_retry += h[0];
 		scale *= 4;
-		const Caps c = gwf_caps_for(h[1], scale, dg->g.mixed != 0);
+		const Caps c = gwf_caps_for(h[1], scale, g.mixed != 0);
 		const size_t wb = gwf_ws_bytes(c);
 		if (wb > ws_bytes || (c.x >= (1 << 25) && scale > (1 << 12))) {
 			gwf_set_error("gwf_ed_run_noalloc: %d reads need workspaces of %zu bytes; the scratch holds %zu", h[0], wb,
@@ -140,7 +142,7 @@ extern "C" int gwf_ed_run_noalloc(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t v1
 		}
 		const i32 per = (i32)std::min<size_t>(ws_bytes / wb, (size_t)h[0]);
 		for (i32 o = 0; o < h[0] && e == hipSuccess; o += per) {
-			e = gwf_launch_plain_uniform(dg->g, rv, list[cur] + o, 0, std::min(per, h[0] - o), c.x, ws, ed, status,
+			e = gwf_launch_plain_uniform(g, rv, list[cur] + o, 0, std::min(per, h[0] - o), c.x, ws, ed, status,
 			                             nullptr, nullptr);
 			st.n_launch += 1;
 		}

@@ -31,6 +31,7 @@ struct gwf_dgraph_s {
 	DGraph g;
 	void *mem = nullptr;
 	u32 n_vtx = 0;
+	u32 max_vlen = 0;
 	uint64_t gen = 0;
 };
 
@@ -42,3 +43,8 @@ struct gwf_dreads_s {
 	uint64_t gen = 0;
 	std::vector<i32> ql, v0;
 };
+
+// The device graph for a run of dr on dg: dg's, with the first state layout
+// that holds dg's vertex count and vertex lengths and dr's read lengths
+// (gwf_pick_layout); GWF_ERR_ARG (with `what` in the error text) if none does.
+int gwf_run_graph(const gwf_dgraph_t *dg, const gwf_dreads_t *dr, const char *what, DGraph *out);

@@ -372,7 +372,7 @@ int gwf_dgraph_upload(const gwf_graph_t *g, gwf_dgraph_t **out) {
 	if (!g || !out) { set_err("gwf_dgraph_upload: NULL argument"); return GWF_ERR_ARG; }
 	*out = nullptr;
 	if (!g->aux && g->n_arc) { set_err("gwf_dgraph_upload: graph not indexed (call gwf_ed_index)"); return GWF_ERR_ARG; }
-	if (g->n_vtx == 0 || g->n_vtx >= GWF_MAX_VTX) { set_err("gwf_dgraph_upload: n_vtx %u out of range", g->n_vtx); return GWF_ERR_ARG; }
+	if (g->n_vtx == 0 || g->n_vtx > gwf_layout_max_vtx(3)) { set_err("gwf_dgraph_upload: n_vtx %u out of range", g->n_vtx); return GWF_ERR_ARG; }
 	int rc = device_ready();
 	if (rc) return rc;
 	const u32 nv = g->n_vtx, na = g->n_arc;
@@ -380,12 +380,18 @@ int gwf_dgraph_upload(const gwf_graph_t *g, gwf_dgraph_t **out) {
 	std::vector<i32> in_ol(nv, -1);
 	bool mixed = false;
 	uint64_t total = 0;
+	u32 max_vlen = 0;
 	for (u32 v = 0; v < nv; ++v) {
 		if (g->len[v] == 0 || g->len[v] > (u32)GWF_MAX_VLEN) {
 			set_err("gwf_dgraph_upload: vertex %u has length %u (supported: 1..%d)", v, g->len[v], GWF_MAX_VLEN);
 			return GWF_ERR_ARG;
 		}
 		total += g->len[v];
+		max_vlen = std::max(max_vlen, g->len[v]);
+	}
+	if (gwf_pick_layout(nv, max_vlen, 0) < 0) {
+		set_err("gwf_dgraph_upload: no state layout holds %u vertices with vertex length %u", nv, max_vlen);
+		return GWF_ERR_ARG;
 	}
 	if (total + kSeqPad >= (1ull << 32)) { set_err("gwf_dgraph_upload: graph sequence exceeds 4 GiB"); return GWF_ERR_ARG; }
 	std::vector<uint4> arcs(na ? na : 1, make_uint4(0, 0, 0, 0));
@@ -443,7 +449,8 @@ int gwf_dgraph_upload(const gwf_graph_t *g, gwf_dgraph_t **out) {
 	dg->g.n_vtx = nv;
 	dg->g.total_len = (u32)total;
 	dg->g.mixed = mixed ? 1u : 0u;
-	dg->g.pad = 0;
+	dg->g.layout = 0;
+	dg->max_vlen = max_vlen;
 	dg->n_vtx = nv;
 	dg->gen = gwf_next_generation();
 	if (e != hipSuccess) {
@@ -533,6 +540,31 @@ void gwf_dreads_free(gwf_dreads_t *dr) {
 	delete dr;
 }
 
+extern "C++" int gwf_run_graph(const gwf_dgraph_t *dg, const gwf_dreads_t *dr, const char *what, DGraph *out) {
+	i32 maxql = 0;
+	for (i32 q : dr->ql) maxql = std::max(maxql, q);
+	int l = gwf_pick_layout(dg->n_vtx, dg->max_vlen, maxql);
+	// GWF_KEY_LAYOUT=n: layout n instead of the first that fits (tests)
+	const int64_t force = env_int("GWF_KEY_LAYOUT", -1);
+	if (force >= 0) {
+		l = (int)force;
+		if (force >= GWF_N_LAYOUTS || dg->n_vtx > gwf_layout_max_vtx(l) || dg->max_vlen > gwf_layout_max_vlen(l) ||
+		    maxql > gwf_layout_max_ql(l)) {
+			set_err("%s: state layout %lld (GWF_KEY_LAYOUT) does not hold %u vertices, vertex length %u and read "
+			        "length %d", what, (long long)force, dg->n_vtx, dg->max_vlen, maxql);
+			return GWF_ERR_ARG;
+		}
+	}
+	if (l < 0) {
+		set_err("%s: no state layout holds %u vertices, vertex length %u and read length %d together", what, dg->n_vtx,
+		        dg->max_vlen, maxql);
+		return GWF_ERR_ARG;
+	}
+	*out = dg->g;
+	out->layout = (u32)l;
+	return GWF_OK;
+}
+
 namespace {
 
 struct RunAcc {  // statistics of one gwf_ed_run call
@@ -616,7 +648,7 @@ bool timed_launch(RunCtx &cx, RunAcc &acc, const std::function<hipError_t()> &la
 // its arena after its previous launch (stream order), so launches on
 // different streams overlap.  A read whose workspace alone exceeds an arena
 // gets a batch (and an arena) of its own.
-bool run_plain(RunCtx &cx, RunAcc &acc, const gwf_dgraph_t *dg, const DReads &r, const std::vector<i32> &qlh,
+bool run_plain(RunCtx &cx, RunAcc &acc, const DGraph &g, const DReads &r, const std::vector<i32> &qlh,
                const std::vector<i32> &todo, int scale, size_t budget) {
 	struct Batch {
 		size_t pos;
@@ -629,7 +661,7 @@ bool run_plain(RunCtx &cx, RunAcc &acc, const gwf_dgraph_t *dg, const DReads &r,
 	std::vector<size_t> wsb(nt);
 	size_t total = 0;
 	for (size_t j = 0; j < nt; ++j) {
-		const Caps c = caps_for(qlh[todo[j]], scale, dg->g.mixed != 0);
+		const Caps c = caps_for(qlh[todo[j]], scale, g.mixed != 0);
 		wa[j] = c.x;
 		wsb[j] = gwf_ws_bytes(c);
 		total += wsb[j];
@@ -659,7 +691,7 @@ bool run_plain(RunCtx &cx, RunAcc &acc, const gwf_dgraph_t *dg, const DReads &r,
 		for (int k = 0; k < used && e == hipSuccess; ++k) e = hipStreamWaitEvent(cx.pst[k], cx.ev_fork, 0);
 		for (const Batch &x : bt) {
 			if (e != hipSuccess) break;
-			e = gwf_launch_plain(dg->g, r, cx.idx + x.pos, x.nb, cx.woff + x.pos, cx.wa + x.pos, cx.pscr[x.stream],
+			e = gwf_launch_plain(g, r, cx.idx + x.pos, x.nb, cx.woff + x.pos, cx.wa + x.pos, cx.pscr[x.stream],
 			                     cx.ed, cx.status, cx.cnt, cx.pst[x.stream]);
 		}
 		for (int k = 0; k < used && e == hipSuccess; ++k) {
@@ -670,43 +702,12 @@ bool run_plain(RunCtx &cx, RunAcc &acc, const gwf_dgraph_t *dg, const DReads &r,
 	}, nullptr, (int)bt.size());
 }
 
-// Persistent kernel over one host-ordered queue (the autotuner's runs).
-bool run_persistent(RunCtx &cx, RunAcc &acc, const gwf_dgraph_t *dg, const DReads &r, const std::vector<i32> &order,
-                    const PersistCfg &cfg) {
-	const i32 n = (i32)order.size();
-	if (n == 0) return true;
-	const size_t ws = gwf_ws_bytes(cfg.hbm);
-	ACC(grow(&cx.scratch, &cx.scratch_bytes, ws * (size_t)cfg.wgs));
-	ACC(hipMemcpyAsync(cx.idx, order.data(), 4 * (size_t)n, hipMemcpyHostToDevice, cx.s));
-	ACC(gwf_launch_fill(cx.boff + 16, 1, 0, cx.s));
-	ACC(gwf_launch_fill(cx.boff + 17, 1, n, cx.s));
-	ACC(gwf_launch_fill(cx.head + 16, 1, 0, cx.s));
-	Heartbeat hb;
-	hb.period_ms = (int)env_int("GWF_HEARTBEAT_MS", 0);
-	if (hb.period_ms > 0) {
-		if ((size_t)cfg.wgs > cx.beat_n) {
-			if (cx.beat) (void)hipHostFree(cx.beat);
-			cx.beat = nullptr;
-			cx.beat_n = 0;
-			ACC(hipHostMalloc((void **)&cx.beat, sizeof(GwfBeat) * cfg.wgs, hipHostMallocCoherent | hipHostMallocMapped));
-			cx.beat_n = cfg.wgs;
-		}
-		for (i32 w = 0; w < cfg.wgs; ++w) cx.beat[w].read = -1, cx.beat[w].beat = 0;
-		hb.beat = cx.beat;
-		hb.n = cfg.wgs;
-		hb.what = "persistent";
-	}
-	return timed_launch(cx, acc, [&] {
-		return gwf_launch_persistent(dg->g, r, cx.idx, cx.boff + 16, 0, cx.head + 16, cfg, cx.scratch, ws, cx.ed,
-		                             cx.status, cx.cnt, hb.beat, cx.s);
-	}, &hb);
-}
-
 // The binned run, all on the device: probe every read's prefix, order the
 // reads into per-bin longest-first queues, then one persistent launch per
 // bin with that bin's tuned configuration (on forked streams when their
 // workspaces fit the budget, so each bin's tail overlaps the next bin).
 struct BinnedPlan {
+	DGraph g;  // the graph, with the run's key layout
 	PersistCfg probe;
 	PersistCfg bin[GWF_MAX_BINS];
 	BinBounds bb;
@@ -748,9 +749,9 @@ hipError_t enqueue_binned(RunCtx &cx, const gwf_dgraph_t *dg, const gwf_dreads_t
 	ENQ(gwf_launch_fill(cx.boff + 17, 1, p.n, cx.s));
 	ENQ(gwf_launch_fill(cx.head + 16, 1, 0, cx.s));
 	if (p.direct)  // reads in input order from the same one-range queue the probe uses
-		return gwf_launch_persistent(dg->g, rr, nullptr, cx.boff + 16, 0, cx.head + 16, p.bin[0], cx.scr[0],
+		return gwf_launch_persistent(p.g, rr, nullptr, cx.boff + 16, 0, cx.head + 16, p.bin[0], cx.scr[0],
 		                             gwf_ws_bytes(p.bin[0].hbm), cx.ed, cx.status, cx.cnt, p.beat, cx.s);
-	ENQ(gwf_launch_persistent(dg->g, rp, nullptr, cx.boff + 16, 0, cx.head + 16, p.probe, cx.scratch,
+	ENQ(gwf_launch_persistent(p.g, rp, nullptr, cx.boff + 16, 0, cx.head + 16, p.probe, cx.scratch,
 	                          gwf_ws_bytes(p.probe.hbm), cx.pe, cx.ps, cx.cnt,
 	                          p.beat ? p.beat + (size_t)p.bb.n_bins * p.beat_stride : nullptr, cx.s));
 	ENQ(gwf_launch_binning(cx.pe, cx.ps, dr->r.ql, p.n, p.bb, cx.key, cx.hist, cx.ord, cx.boff, cx.head, cx.s));
@@ -761,7 +762,7 @@ hipError_t enqueue_binned(RunCtx &cx, const gwf_dgraph_t *dg, const gwf_dreads_t
 	for (int b = p.bb.n_bins - 1; b >= 0; --b) {  // costliest bin first
 		hipStream_t st = p.concurrent ? cx.sb[b] : cx.s;
 		u64 *scr = p.concurrent ? cx.scr[b] : cx.scr[0];
-		ENQ(gwf_launch_persistent(dg->g, rr, cx.ord, cx.boff, b, cx.head, p.bin[b], scr,
+		ENQ(gwf_launch_persistent(p.g, rr, cx.ord, cx.boff, b, cx.head, p.bin[b], scr,
 		                          gwf_ws_bytes(p.bin[b].hbm), cx.ed, cx.status, cx.cnt,
 		                          p.beat ? p.beat + (size_t)b * p.beat_stride : nullptr, st));
 	}
@@ -775,15 +776,16 @@ hipError_t enqueue_binned(RunCtx &cx, const gwf_dgraph_t *dg, const gwf_dreads_t
 	return hipSuccess;
 }
 
-bool run_binned(RunCtx &cx, RunAcc &acc, const gwf_dgraph_t *dg, const gwf_dreads_t *dr, i32 v1, i32 maxql,
+bool run_binned(RunCtx &cx, RunAcc &acc, const gwf_dgraph_t *dg, const DGraph &g, const gwf_dreads_t *dr, i32 v1, i32 maxql,
                 size_t budget, bool graph_ok) {
 	const i32 n = dr->r.n;
 	u64 tv = 0;
 	const gwf_tune_t t = tune_current(&tv);
 	BinnedPlan p;
+	p.g = g;
 	p.n = n;
 	p.v1 = v1;
-	const bool mixed = dg->g.mixed != 0;
+	const bool mixed = g.mixed != 0;
 	p.probe = make_cfg(cx, 1, 256, 8, std::min(maxql, kProbeLen), mixed);
 	p.bb.n_bins = t.n_bins;
 	size_t ws_sum = 0, ws_max = 0;
@@ -852,7 +854,7 @@ bool run_binned(RunCtx &cx, RunAcc &acc, const gwf_dgraph_t *dg, const gwf_dread
 	                              (uintptr_t)dr->ql_probe, (uintptr_t)n, (uintptr_t)tv, (uintptr_t)maxql, (uintptr_t)(v1 + 1),
 	                              (uintptr_t)p.concurrent, (uintptr_t)p.direct, (uintptr_t)cx.ed, (uintptr_t)cx.status, (uintptr_t)cx.cnt,
 	                              (uintptr_t)cx.scratch, (uintptr_t)cx.pe, (uintptr_t)cx.ps, (uintptr_t)cx.key,
-	                              (uintptr_t)cx.ord};
+	                              (uintptr_t)cx.ord, (uintptr_t)p.g.layout};
 	for (int b = 0; b < GWF_MAX_BINS; ++b) key.push_back((uintptr_t)cx.scr[b]);
 	for (int b = 0; b < t.n_bins; ++b) {
 		key.push_back((uintptr_t)p.bin[b].nw);
@@ -901,6 +903,8 @@ int gwf_ed_run_timed(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t v1, uint32_t ma
 	const i32 n = dr->r.n;
 	for (i32 i = 0; i < n; ++i)
 		if ((u32)dr->v0[i] >= dg->n_vtx) { set_err("gwf_ed_run: read %d has v0 %d >= n_vtx %u", i, dr->v0[i], dg->n_vtx); return GWF_ERR_ARG; }
+	DGraph g;
+	if (int rc = gwf_run_graph(dg, dr, "gwf_ed_run", &g)) return rc;
 	RunAcc acc;
 	memset(&acc.st, 0, sizeof(acc.st));
 	if (n == 0) { if (stats) *stats = acc.st; return GWF_OK; }
@@ -928,7 +932,7 @@ int gwf_ed_run_timed(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t v1, uint32_t ma
 	if (kernel == GWF_KERNEL_PERSISTENT) {
 		i32 maxql = 0;
 		for (i32 q : dr->ql) maxql = std::max(maxql, q);
-		ok = run_binned(cx, acc, dg, dr, v1, maxql, budget, true);
+		ok = run_binned(cx, acc, dg, g, dr, v1, maxql, budget, true);
 		if (ok) ok = hipMemcpy(status.data(), cx.status, 4 * (size_t)n, hipMemcpyDeviceToHost) == hipSuccess;
 		for (i32 i = 0; i < n && ok; ++i)
 			if (status[i] == GWF_ST_OVERFLOW) todo.push_back(i);
@@ -945,7 +949,7 @@ int gwf_ed_run_timed(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t v1, uint32_t ma
 	while (ok && rc == GWF_OK && !todo.empty()) {
 		DReads rv = dr->r;
 		rv.v1 = v1;
-		ok = run_plain(cx, acc, dg, rv, dr->ql, todo, scale, plain_budget);
+		ok = run_plain(cx, acc, g, rv, dr->ql, todo, scale, plain_budget);
 		if (!ok) break;
 		ok = hipMemcpy(status.data(), cx.status, 4 * (size_t)n, hipMemcpyDeviceToHost) == hipSuccess;
 		std::vector<i32> again;
@@ -1032,6 +1036,8 @@ int gwf_autotune(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t max_evals, gwf_tune
 		if (rc) return rc;
 		i32 maxql = 0;
 		for (i32 q : dr->ql) maxql = std::max(maxql, q);
+		DGraph g;
+		if (int rc2 = gwf_run_graph(dg, dr, "gwf_autotune", &g)) return rc2;
 		size_t free_b = 0, tot_b = 0;
 		if (hipMemGetInfo(&free_b, &tot_b) != hipSuccess) { set_err("hipMemGetInfo failed"); return GWF_ERR_HIP; }
 		const size_t budget = (size_t)env_int("GWF_BUDGET_BYTES", (int64_t)std::min<size_t>((free_b + ctx_held(cx)) / 2, (size_t)32 << 30));
@@ -1048,7 +1054,7 @@ int gwf_autotune(gwf_dgraph_t *dg, gwf_dreads_t *dr, int32_t max_evals, gwf_tune
 				g_tune_version += 1;
 			}
 			const double before = acc.st.kernel_ms;
-			if (!run_binned(cx, acc, dg, dr, -1, maxql, budget, false)) return -1.0;
+			if (!run_binned(cx, acc, dg, g, dr, -1, maxql, budget, false)) return -1.0;
 			if (hipMemcpy(status.data(), cx.status, 4 * (size_t)n, hipMemcpyDeviceToHost) != hipSuccess) return -1.0;
 			i32 over = 0;
 			for (i32 x : status) over += x != GWF_ST_OK;

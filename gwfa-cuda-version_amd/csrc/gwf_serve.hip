@@ -147,6 +147,8 @@ struct gwf_serve_s {
 	// configuration
 	int dev = 0;
 	const gwf_dgraph_t *dg = nullptr;
+	DGraph g{};      // dg's graph with the server's state layout
+	i32 max_ql = 0;  // the longest request that layout holds
 	gwf_serve_opts_t o{};
 	PersistCfg cfg{};
 	double ns_per_tick = 10.0;
@@ -231,7 +233,7 @@ void gwf_serve_s::finish(std::unique_ptr<Req> r, const SrvResult *res, int statu
 // Publish the head of `ready` if a slot, budget and staging space are free.
 bool gwf_serve_s::admit_one(int64_t now) {
 	Req &r = *ready.front();
-	const Caps hc = gwf_caps_for(r.ql, r.scale, dg->g.mixed != 0);
+	const Caps hc = gwf_caps_for(r.ql, r.scale, g.mixed != 0);
 	const u64 q_len = round_up((u64)r.ql, 16) + 16;
 	const u64 ws_off = round_up(round_up((u64)r.ql, 16) + kSeqPad, 256);
 	const u64 d_len = ws_off + gwf_ws_bytes(hc);
@@ -336,7 +338,7 @@ bool gwf_serve_s::device_check(int64_t now) {
 	}
 	if (!running && !inflight.empty()) {
 		SrvArgs a;
-		a.g = dg->g;
+		a.g = g;
 		a.ring = ring_d;
 		a.res = res_d;
 		a.ctl = ctl_d;
@@ -525,6 +527,15 @@ int gwf_serve_start(gwf_dgraph_t *dg, const gwf_serve_opts_t *opts, gwf_serve_t 
 	}
 	s->o = o;
 	s->dg = dg;
+	// the state layout that holds the graph and the longest requests
+	s->g = dg->g;
+	s->max_ql = -1;
+	for (int l = 0; l < GWF_N_LAYOUTS; ++l)
+		if (dg->n_vtx <= gwf_layout_max_vtx(l) && dg->max_vlen <= gwf_layout_max_vlen(l) &&
+		    gwf_layout_max_ql(l) > s->max_ql) {  // the upload checked that one layout holds the graph
+			s->g.layout = (u32)l;
+			s->max_ql = gwf_layout_max_ql(l);
+		}
 	hipError_t e = hipGetDevice(&s->dev);
 	int khz = 0, n_cu = 0;
 	if (e == hipSuccess) e = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, s->dev);
@@ -587,8 +598,9 @@ int gwf_serve_submit_v1(gwf_serve_t *s, int32_t n, const char *seqs, const int64
 	std::vector<std::unique_ptr<Req>> batch;
 	batch.reserve((size_t)n);
 	for (i32 i = 0; i < n; ++i) {
-		if (ql[i] < 0 || ql[i] > GWF_MAX_QLEN) {
-			gwf_set_error("gwf_serve_submit: request %d has length %d (supported: 0..%d)", i, ql[i], GWF_MAX_QLEN);
+		if (ql[i] < 0 || ql[i] > s->max_ql) {
+			gwf_set_error("gwf_serve_submit: request %d has length %d (supported on this graph: 0..%d)", i, ql[i],
+			              s->max_ql);
 			return GWF_ERR_ARG;
 		}
 		const i32 v = v0 ? v0[i] : 0;
