@@ -128,7 +128,14 @@ def serve_bench(gw, args):
                       "graph_seed": cfg["graph"]["seed"], "read_seed": cfg["reads"]["seed"],
                       "arrivals": "Poisson, rate = fraction x saturated reads/s, seed 5"}}
     mism = 0
-    with gw.Server(dg, state_budget_bytes=args.budget << 20) as srv:
+    shape = {}
+    if args.serve_nw:
+        shape["nw"] = args.serve_nw
+    if args.serve_lds:
+        shape["lds_states"] = args.serve_lds
+    out["config"]["shape"] = {"nw": args.serve_nw or 4, "lds_states": args.serve_lds or 1024}
+    bounds = [0, 2000, 5000, 10000, 1 << 30]
+    with gw.Server(dg, state_budget_bytes=args.budget << 20, **shape) as srv:
         # warm-up, then the saturated rate: every request released at once
         srv.submit(reads.subset(np.arange(min(n, 2000))))
         srv.collect(min(n, 2000))
@@ -159,7 +166,12 @@ def serve_bench(gw, args):
                    "p50_ms": round(float(np.percentile(lat, 50)), 3),
                    "p99_ms": round(float(np.percentile(lat, 99)), 3),
                    "service_p50_ms": round(float(np.percentile(res["service_ns"], 50)) / 1e6, 3),
+                   "service_p99_ms": round(float(np.percentile(res["service_ns"], 99)) / 1e6, 3),
                    "queue_p99_ms": round(float(np.percentile((res["t_admit_ns"] - res["t_arrival_ns"]) / 1e6, 99)), 3)}
+            qs = ql[idx]
+            row["p99_ms_by_length"] = {
+                f"{lo}-{hi if hi < (1 << 30) else 'max'}": round(float(np.percentile(lat[(qs >= lo) & (qs < hi)], 99)), 3)
+                for lo, hi in zip(bounds[:-1], bounds[1:]) if ((qs >= lo) & (qs < hi)).any()}
             sweep.append(row)
             log(f"load {f}: {row}")
         out["sweep"] = sweep
@@ -199,6 +211,8 @@ def main():
     ap.add_argument("--rates", default="0.25,0.5,0.75,0.9,1.0", help="--serve: loads as fractions of saturation")
     ap.add_argument("--serve-seconds", type=float, default=1.5, help="--serve: arrival window per load")
     ap.add_argument("--budget", type=int, default=4096, help="--serve: device state budget, MiB")
+    ap.add_argument("--serve-nw", type=int, default=0, help="--serve: waves per request (default: the server's, 4)")
+    ap.add_argument("--serve-lds", type=int, default=0, help="--serve: LDS states per request (default: the server's, 1024)")
     ap.add_argument("--traffic", default=None,
                     help="JSON with calibrated HBM bytes per step from rocprofv3 --pmc (default: "
                          "profiles/traffic_<kernel>_<workload>.json when present)")

@@ -15,7 +15,8 @@ SO = os.path.join(HERE, "liboracle.so")
 
 class OracleStats(C.Structure):
     _fields_ = [("steps", C.c_int64), ("states", C.c_int64), ("chars", C.c_int64),
-                ("max_na", C.c_int64), ("a_events", C.c_int64), ("crossings", C.c_int64)]
+                ("max_na", C.c_int64), ("a_events", C.c_int64), ("crossings", C.c_int64),
+                ("vis_lossy", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -22,7 +22,10 @@
  *
  * Pinning: oracle_dp_ed() below is an independent textbook DP over graph
  * positions; tests compare the two on random graphs/reads and on the shipped
- * fixtures.
+ * fixtures: equal for v1 < 0 whenever no visited-set claim was refused to an
+ * arc of another overlap (always so when every vertex's in-arcs share one
+ * overlap), never below it otherwise; for v1 >= 0 equal wherever the
+ * vertex-end insertion skip cannot fire at v1, never below it otherwise.
  */
 #define _GNU_SOURCE
 #include <stdlib.h>
@@ -67,15 +70,17 @@ typedef struct {
 	uint64_t *intv0, *intv1; size_t m_intv0, m_intv1; int64_t n_intv;
 	uint64_t *swap0, *swap1; size_t m_swap0, m_swap1; int64_t n_swap;
 	uint64_t *ha; size_t ha_cap, ha_n;                /* visited set, 0 = empty */
+	int32_t *hol;                                     /* overlap of the arc that claimed each key */
 	int64_t nc;                                       /* chars compared */
 	int64_t na_events, ncross;
+	int64_t nlossy;                                   /* claims refused to a different overlap */
 } obuf_t;
 
 static void ob_free(obuf_t *b)
 {
 	free(b->a_vd); free(b->a_k); free(b->a_xo); free(b->A); free(b->B); free(b->ooo);
 	free(b->scr); free(b->tmp0); free(b->tmp1); free(b->intv0); free(b->intv1);
-	free(b->swap0); free(b->swap1); free(b->ha);
+	free(b->swap0); free(b->swap1); free(b->ha); free(b->hol);
 }
 
 static void ob_reserve_a(obuf_t *b, size_t n)
@@ -136,7 +141,11 @@ static void or_tmp_push(obuf_t *b, uint64_t vd)
 
 /* static_set64_put, gwf-ed.cu:1020-1062, as an exact growable set.  Key 0 is
  * reported present, as in the reference (it is (v0, query position 0), the
- * start itself). */
+ * start itself).  Not in the reference: the set also keeps the overlap of the
+ * arc that claimed each key and counts refused claims that came with another
+ * overlap (nlossy).  Only such a refusal drops a state that is not a copy of
+ * one already reached, so a run with none must give the exact distance
+ * (checked against the DP by the tests). */
 static uint64_t or_hash64(uint64_t x)
 {
 	x ^= x >> 33; x *= 0xff51afd7ed558ccdULL;
@@ -144,27 +153,28 @@ static uint64_t or_hash64(uint64_t x)
 	x ^= x >> 33;
 	return x;
 }
-static int or_set_put(obuf_t *b, uint64_t key)
+static int or_set_put(obuf_t *b, uint64_t key, int32_t ol)
 {
 	size_t i, mask;
-	if (key == 0) return 0;
+	if (key == 0) { b->nlossy += ol != 0; return 0; }
 	if ((b->ha_n + 1) * 2 > b->ha_cap) { /* grow, rehash */
 		size_t nc = b->ha_cap ? b->ha_cap << 1 : 1024, j;
 		uint64_t *nh = calloc(nc, sizeof(uint64_t));
-		if (!nh) abort();
+		int32_t *no = calloc(nc, sizeof(int32_t));
+		if (!nh || !no) abort();
 		for (j = 0; j < b->ha_cap; ++j) {
 			uint64_t x = b->ha[j];
 			if (x == 0) continue;
 			for (i = or_hash64(x) & (nc - 1); nh[i]; i = (i + 1) & (nc - 1)) {}
-			nh[i] = x;
+			nh[i] = x, no[i] = b->hol[j];
 		}
-		free(b->ha);
-		b->ha = nh, b->ha_cap = nc;
+		free(b->ha); free(b->hol);
+		b->ha = nh, b->hol = no, b->ha_cap = nc;
 	}
 	mask = b->ha_cap - 1;
 	for (i = or_hash64(key) & mask;; i = (i + 1) & mask) {
-		if (b->ha[i] == key) return 0;
-		if (b->ha[i] == 0) { b->ha[i] = key; ++b->ha_n; return 1; }
+		if (b->ha[i] == key) { b->nlossy += b->hol[i] != ol; return 0; }
+		if (b->ha[i] == 0) { b->ha[i] = key, b->hol[i] = ol; ++b->ha_n; return 1; }
 	}
 }
 
@@ -452,7 +462,7 @@ static int or_ed_extend(obuf_t *b, const gwf_graph_t *g, int32_t ql, const char 
 			for (j = 0; j < nv; ++j) {
 				uint32_t w = (uint32_t)g->arc[ov + j].a;
 				int32_t ol = g->arc[ov + j].o;
-				int absent = or_set_put(b, (uint64_t)w << 32 | (uint32_t)(ii + 1));
+				int absent = or_set_put(b, (uint64_t)w << 32 | (uint32_t)(ii + 1), ol);
 				++b->ncross;
 				if (q[ii + 1] == g->seq[w][ol]) {
 					++n_ext;
@@ -513,7 +523,7 @@ int32_t oracle_gwf_ed(const gwf_graph_t *g, int32_t ql, const char *q, int32_t v
 	if (st) {
 		st->steps += steps, st->states += states, st->chars += b.nc;
 		if (max_na > st->max_na) st->max_na = max_na;
-		st->a_events += b.na_events, st->crossings += b.ncross;
+		st->a_events += b.na_events, st->crossings += b.ncross, st->vis_lossy += b.nlossy;
 	}
 	ob_free(&b);
 	return end_v >= 0 ? s : -1;
@@ -659,7 +669,7 @@ static void *or_worker(void *arg)
 	pthread_mutex_lock(&p->mu);
 	p->st.steps += st.steps, p->st.states += st.states, p->st.chars += st.chars;
 	if (st.max_na > p->st.max_na) p->st.max_na = st.max_na;
-	p->st.a_events += st.a_events, p->st.crossings += st.crossings;
+	p->st.a_events += st.a_events, p->st.crossings += st.crossings, p->st.vis_lossy += st.vis_lossy;
 	pthread_mutex_unlock(&p->mu);
 	return 0;
 }
@@ -684,7 +694,7 @@ int oracle_batch(const gwf_graph_t *g, int32_t n, const char *seqs, const int64_
 	if (st) {
 		st->steps += p.st.steps, st->states += p.st.states, st->chars += p.st.chars;
 		if (p.st.max_na > st->max_na) st->max_na = p.st.max_na;
-		st->a_events += p.st.a_events, st->crossings += p.st.crossings;
+		st->a_events += p.st.a_events, st->crossings += p.st.crossings, st->vis_lossy += p.st.vis_lossy;
 	}
 	return n_threads;
 }

@@ -24,6 +24,7 @@ typedef struct {
 	int64_t max_na;    /* largest wavefront seen */
 	int64_t a_events;  /* states processed by the A-queue closure */
 	int64_t crossings; /* arc traversals in the closure */
+	int64_t vis_lossy; /* visited-set claims refused to an arc of another overlap (not in the reference) */
 } oracle_stats_t;
 
 /* gwf_ed with exact semantics (signed max_k, growable buffers, no drops).

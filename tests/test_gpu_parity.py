@@ -428,3 +428,21 @@ def test_mixed_in_overlaps_order_sensitive_cases(gpu, orc):
                     dg.free()
                 assert got.tolist() == [want], (seqs, arcs, q, kernel)
     assert found == 25
+
+
+def test_mixed_claim_order_hand_cases(gpu, orc):
+    """The hand-derived claim-order cases of test_oracle (which arc wins a
+    visited key: CSR order within a vertex, first in first out across the
+    closure queue, the key claimed before the first-base test)."""
+    _say("case: mixed in-overlaps, hand-derived claim-order cases")
+    from test_oracle import _claim_order_cases
+    for g, q, want in _claim_order_cases(gpu):
+        for kernel in KERNELS:
+            dg, dr = gpu.DeviceGraph(g), gpu.DeviceReads(gpu.Reads.from_list([q]))
+            try:
+                got, _ = gpu.ed_run(dg, dr, kernel=kernel)
+            finally:
+                dr.free()
+                dg.free()
+            assert got.tolist() == [want], (g.arcs(), kernel)
+        assert gpu.gwf_ed(g, q) == want

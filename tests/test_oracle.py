@@ -246,6 +246,134 @@ def test_mixed_in_overlaps_make_the_result_order_dependent(gw, orc):
     assert (orc.ed(g, q), orc.ed(g2, q), orc.dp(g, q)) == (12, 11, 11)
 
 
+# ---- mixed in-overlaps.  The visited set refuses a second claim on a key
+# (w, i+1) (gwf-ed.cu:1196-1212).  A refusal to an arc of the same overlap
+# drops a copy of a state already reached, which changes nothing; only a
+# refusal to an arc of another overlap (counted by the oracle as vis_lossy)
+# can drop a state the exact distance needs.  So: no such refusal -> the
+# exact DP; otherwise never below it.  Which claim wins is fixed by the
+# reference's order: arcs of one vertex in CSR order (the stable sort of
+# gwf_ed_index keeps input order between parallel arcs), the A-queue first in,
+# first out, and the key claimed before the first-base test.
+
+def _mixed_case(gw, rng, dag):
+    nv = rng.randint(1, 12)
+    seqs = ["".join(rng.choice("ACGT"[:rng.randint(2, 4)]) for _ in range(rng.randint(1, 3 if rng.random() < 0.4 else 10)))
+            for _ in range(nv)]
+    arcs = set()
+    for v in range(nv):
+        for _ in range(rng.randint(0, 3)):
+            if dag:
+                if v + 1 >= nv:
+                    break
+                w = rng.randint(v + 1, nv - 1)
+            else:
+                w = rng.randrange(nv)
+            arcs.add((v, w, rng.randrange(len(seqs[w]))))
+    g = gw.Graph.from_lists(seqs, sorted(arcs))
+    g.index()
+    return g, rand_read(rng, rng.randint(0, 25))
+
+
+@pytest.mark.parametrize("dag", [True, False])
+def test_mixed_in_overlaps_exact_without_lossy_claims(gw, orc, dag):
+    rng = random.Random(41 + dag)
+    lossy = above = 0
+    for it in range(2000):
+        g, q = _mixed_case(gw, rng, dag)
+        st = {}
+        e, d = orc.ed(g, q, stats=st), orc.dp(g, q)
+        if st["vis_lossy"]:
+            lossy += 1
+            above += e > d
+            assert e >= d, (it, g.arcs(), q)
+        else:
+            assert e == d, (it, g.arcs(), q)
+    assert lossy > 300 and above > 20   # both kinds of run occur
+
+
+def _claim_order_cases(gw):
+    """(graph, query, the reference's answer) derived by hand from
+    gwf-ed.cu:1186-1212.  1: one vertex "AC" with two arcs into "GT", of
+    overlaps 0 and 1; q = "ACT".  At the end of "AC" (query position 1) the
+    first arc in CSR order claims (w, 2): with overlap 0 its first base G
+    mismatches T, so only a mismatch/deletion state enters w and the claim of
+    the overlap-1 arc (which would match T and end at 0) is refused: answer 1.
+    With the arcs the other way round the overlap-1 arc claims first: 0.
+    2: "A" -> "C" (vertex 1), "A" -> "C" (vertex 2), both -> "GT" with overlaps
+    o1 and o2; q = "ACT".  The two "C" states enter the A-queue in CSR order
+    (1 before 2) and, first in first out, vertex 1 claims (3, 2) first: 1 when
+    o1 = 0, 0 when o1 = 1.  The exact distance is 0 in all four."""
+    out = []
+    for arcs, want in (([(0, 1, 0), (0, 1, 1)], 1), ([(0, 1, 1), (0, 1, 0)], 0)):
+        out.append((gw.Graph.from_lists(["AC", "GT"], arcs), "ACT", want))
+    for o1, o2, want in ((0, 1, 1), (1, 0, 0)):
+        out.append((gw.Graph.from_lists(["A", "C", "C", "GT"], [(0, 1, 0), (0, 2, 0), (1, 3, o1), (2, 3, o2)]),
+                    "ACT", want))
+    for g, _, _ in out:
+        g.index()
+    return out
+
+
+def test_mixed_claim_order_hand_cases(gw, orc):
+    for g, q, want in _claim_order_cases(gw):
+        assert orc.ed(g, q) == want
+        assert orc.dp(g, q) == 0
+
+
+# Plausible mistakes in the oracle's closure crossing (gwf_oracle.c, the
+# restatement of gwf-ed.cu:1186-1212), each planted in a copy compiled here;
+# the checks above must reject every one.
+MIXED_PLANTS = [
+    ("visited key without the query position",
+     "or_set_put(b, (uint64_t)w << 32 | (uint32_t)(ii + 1), ol)", "or_set_put(b, (uint64_t)w << 32, ol)"),
+    ("entry at the vertex start instead of at the overlap",
+     "or_A_push(b, or_gen_vd(w, ii + 1 - ol), ol,", "or_A_push(b, or_gen_vd(w, ii + 1), 0,"),
+    ("a key claimed only by a matching arc",
+     "int absent = or_set_put(b, (uint64_t)w << 32 | (uint32_t)(ii + 1), ol);",
+     "int absent = q[ii + 1] == g->seq[w][ol] ? or_set_put(b, (uint64_t)w << 32 | (uint32_t)(ii + 1), ol) : 1;"),
+    ("the A-queue last in, first out",
+     "odiag_t t = b->A[b->head_A];\n\t++b->head_A, --b->n_A;",
+     "odiag_t t = b->A[b->head_A + b->n_A - 1];\n\t--b->n_A;"),
+]
+
+
+def _compile_plant(orc, tmp_path, name, old, new):
+    import ctypes as C
+    import subprocess
+    src = open(os.path.join(os.path.dirname(orc.__file__), "gwf_oracle.c")).read()
+    assert src.count(old) == 1, name
+    path = tmp_path / f"{name}.c"
+    path.write_text(src.replace(old, new))
+    so = tmp_path / f"{name}.so"
+    subprocess.run(["gcc", "-O1", "-fPIC", "-shared", "-w", f"-I{os.path.dirname(orc.__file__)}", "-o", str(so),
+                    str(path), "-lpthread"], check=True)
+    lib = C.CDLL(str(so))
+    lib.oracle_gwf_ed.argtypes = [C.c_void_p, C.c_int32, C.c_char_p, C.c_int32, C.c_int32, C.c_uint32, C.c_void_p]
+    lib.oracle_gwf_ed.restype = C.c_int32
+    return lib
+
+
+def test_mixed_pin_rejects_planted_mistakes(gw, orc, tmp_path):
+    import ctypes as C
+    rng = random.Random(43)
+    cases = [_mixed_case(gw, rng, dag) for dag in (True, False) for _ in range(600)]
+    hand = _claim_order_cases(gw)
+    for i, (what, old, new) in enumerate(MIXED_PLANTS):
+        lib = _compile_plant(orc, tmp_path, f"mixed{i}", old, new)
+        wrong = 0
+        for g, q in cases:
+            st = orc.OracleStats()
+            qb = q.encode()
+            e = lib.oracle_gwf_ed(C.cast(g.ptr, C.c_void_p), len(qb), qb, 0, -1, 0, C.byref(st))
+            d = orc.dp(g, q)
+            wrong += (e != d) if st.vis_lossy == 0 else (e < d)
+        for g, q, want in hand:
+            qb = q.encode()
+            wrong += lib.oracle_gwf_ed(C.cast(g.ptr, C.c_void_p), len(qb), qb, 0, -1, 0, None) != want
+        assert wrong > 0, f"planted mistake not caught: {what}"
+
+
 # ---- v1 >= 0: the domain where the reference algorithm's end-at-v1 result is
 # exact.  The vertex-end insertion is skipped when every out-arc matches the
 # next query base (gwf-ed.cu:1225).  At any vertex other than v1 that only
@@ -338,21 +466,10 @@ V1_PLANTS = [
 
 def test_v1_pin_rejects_planted_mistakes(gw, orc, tmp_path):
     import ctypes as C
-    import subprocess
-    src = open(os.path.join(os.path.dirname(orc.__file__), "gwf_oracle.c")).read()
     rng = random.Random(31)
     cases = [_v1_domain_case(gw, rng, dag) for dag in (True, False) for _ in range(700)]
     for i, (what, old, new) in enumerate(V1_PLANTS):
-        assert src.count(old) == 1, what
-        path = tmp_path / f"plant{i}.c"
-        path.write_text(src.replace(old, new))
-        so = tmp_path / f"plant{i}.so"
-        subprocess.run(["gcc", "-O1", "-fPIC", "-shared", "-w", f"-I{os.path.dirname(orc.__file__)}", "-o", str(so),
-                        str(path), "-lpthread"], check=True)
-        lib = C.CDLL(str(so))
-        lib.oracle_gwf_ed.argtypes = [C.c_void_p, C.c_int32, C.c_char_p, C.c_int32, C.c_int32, C.c_uint32,
-                                      C.c_void_p]
-        lib.oracle_gwf_ed.restype = C.c_int32
+        lib = _compile_plant(orc, tmp_path, f"v1_{i}", old, new)
         wrong = 0
         for g, _, v1, q in cases:
             qb = q.encode()
