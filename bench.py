@@ -73,7 +73,8 @@ def cpu_model():
 def cpu_baseline(gw, graph, reads, sample, sample_1t, gpu_eds):
     """Oracle (CPU port of gwf_ed): the first `sample` reads on every host core
     this process may use (also the parity check of those reads' EDs), then the
-    first `sample_1t` of them on one thread."""
+    first `sample_1t` of them on one thread (0: as many as take ~8 s on one
+    thread at the rate the multi-thread run measured per thread)."""
     sys.path.insert(0, ROOT)
     import oracle
     idx = np.arange(min(sample, len(reads)))
@@ -83,6 +84,10 @@ def cpu_baseline(gw, graph, reads, sample, sample_1t, gpu_eds):
     eds, st, used = oracle.batch(graph, sub, threads=threads)
     dt = time.perf_counter() - t0
     mism = int((eds != gpu_eds[idx]).sum())
+    if sample_1t <= 0:
+        per_thread = sub.bases() / dt / max(used, 1)
+        cum = np.cumsum(sub.ql.astype(np.int64))
+        sample_1t = max(1, int(np.searchsorted(cum, 8.0 * per_thread)))
     sub1 = reads.subset(np.arange(min(sample_1t, len(idx))))
     t1 = time.perf_counter()
     eds1, _, _ = oracle.batch(graph, sub1, threads=1)
@@ -197,8 +202,8 @@ def main():
     ap.add_argument("--reads", type=int, default=None, help="use only the first N reads")
     ap.add_argument("--cpu-sample", type=int, default=None,
                     help="reads timed on the CPU oracle on all host cores and parity-checked (default: every read)")
-    ap.add_argument("--cpu-sample-1t", type=int, default=1500,
-                    help="of those, reads timed again on one thread (default 1500)")
+    ap.add_argument("--cpu-sample-1t", type=int, default=0,
+                    help="of those, reads timed again on one thread (default 0: about 8 s of them)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true",
                     help="persistent kernel: skip the plain-launch comparison run")
