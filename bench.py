@@ -205,6 +205,7 @@ def main():
     ap.add_argument("--cpu-sample-1t", type=int, default=0,
                     help="of those, reads timed again on one thread (default 0: about 8 s of them)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-pcie", action="store_true", help="skip the host-buffers-to-host-EDs (gwf_ed_batch) run")
     ap.add_argument("--no-plain", action="store_true",
                     help="persistent kernel: skip the plain-launch comparison run")
     ap.add_argument("--tune", default=None, help="persistent kernel: load this bin table (no autotuning)")
@@ -337,6 +338,21 @@ def main():
                                          "gwf_ed_run each, same resident reads); plain = one 64-lane wavefront per "
                                          "read, batches of the budget spread over 16 streams"}
         log(f"plain: {pdt * 1e3:.1f} ms ({pst['n_launch']} launches); persistent/plain {out['vs_plain']['speedup']}x")
+    if not args.no_pcie:
+        # reads in host memory to EDs in host memory (gwf_ed_batch: upload of the
+        # read set, the run, copy-back, free), outside the timed region; not `value`
+        gw.ed_batch(dg, reads, kernel=kernel)
+        tb = time.perf_counter()
+        beds, bst = gw.ed_batch(dg, reads, kernel=kernel)
+        bdt = time.perf_counter() - tb
+        if not np.array_equal(beds, eds):
+            raise SystemExit("bench.py: gwf_ed_batch and gwf_ed_run EDs differ")
+        out["pcie_inclusive"] = {"bases_per_s": round(bases / bdt, 1), "ms": round(bdt * 1e3, 3),
+                                 "kernel_ms": round(bst["kernel_ms"], 3),
+                                 "definition": "one gwf_ed_batch call from host read buffers: read-set upload "
+                                               "over PCIe, the same run (captured anew: new read handle), "
+                                               "EDs copied back, handle freed"}
+        log(f"pcie-inclusive: {bdt * 1e3:.1f} ms per call, {bases / bdt / 1e6:.1f} M bases/s")
     if not args.no_cpu:
         sample = len(reads) if args.cpu_sample is None else args.cpu_sample
         cb, par, ost = cpu_baseline(gw, graph, reads, sample, args.cpu_sample_1t, eds)

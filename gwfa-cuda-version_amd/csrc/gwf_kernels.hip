@@ -307,16 +307,17 @@ struct WaveB {
 	}
 };
 
-// unaligned 8-byte load from a buffer padded past its end: one 16-byte load
-// from the enclosing 8-byte boundary, then a funnel shift.  (Global loads
-// need only dword alignment; the vector type keeps the two halves one
-// instruction.)
-typedef u64 u64x2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ u64 load8u(const uint8_t *p) {
-	const u32 mis = (u32)((uintptr_t)p & 7);
-	const u64x2 w = *(const u64x2 *)(p - mis);  // pointer arithmetic keeps the address space
-	const u32 sh = mis * 8;
-	return sh ? (w.x >> sh) | (w.y << (64 - sh)) : w.x;
+// Unaligned 8-byte load of base[off..off+8) from a buffer padded past its
+// end (base 8-byte aligned and uniform): one 16-byte load from the enclosing
+// 8-byte boundary, addressed as base + 32-bit offset (no 64-bit address
+// arithmetic per lane), then two byte-aligns of the dwords the window covers.
+typedef u32 u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ u64 load8u(const uint8_t *base, u32 off) {
+	const u32x4 w = *(const u32x4 *)(base + (size_t)(off & ~7u));
+	const bool hi = (off & 4u) != 0;
+	const u32 b = off & 3u;
+	const u32 w0 = hi ? w.y : w.x, w1 = hi ? w.z : w.y, w2 = hi ? w.w : w.z;
+	return (u64)__builtin_amdgcn_alignbyte(w2, w1, b) << 32 | __builtin_amdgcn_alignbyte(w1, w0, b);
 }
 
 // first index in a[0..n) with a[i] >= x (lower) or a[i] > x (upper)
@@ -347,13 +348,11 @@ __device__ __forceinline__ i32 extend_wave(bool act, const uint8_t *seq, u32 seq
                            int64_t *chars) {
 	bool more = false;
 	if (act) {
-		const uint8_t *ts = seq + seq_off;
-		const uint8_t *qd = q + d;
 #pragma unroll
 		for (int it = 0; it < 2; ++it) {
 			const i32 avail = max_k - k;
 			if (avail <= 0) break;
-			const u64 x = load8u(ts + k + 1) ^ load8u(qd + k + 1);
+			const u64 x = load8u(seq, seq_off + (u32)(k + 1)) ^ load8u(q, (u32)(d + k + 1));
 			const i32 m = x ? (i32)(__builtin_ctzll(x) >> 3) : 8;
 			if (m >= avail) { *chars += avail; k = max_k; break; }
 			if (m < 8) { *chars += m + 1; k += m; break; }
@@ -369,15 +368,13 @@ __device__ __forceinline__ i32 extend_wave(bool act, const uint8_t *seq, u32 seq
 		lm &= lm - 1;
 		const i32 kk = lane_bcast(k, src), mk_ = lane_bcast(max_k, src), dd = lane_bcast(d, src);
 		const u32 so = (u32)lane_bcast((i32)seq_off, src);
-		const uint8_t *ts = seq + so;
-		const uint8_t *qd = q + dd;
 		i32 base = kk, res;
 		for (;;) {
 			const i32 avail = mk_ - base;
 			const i32 my = (i32)lane * 8;
 			i32 m = 8;
 			if (my < avail) {
-				const u64 x = load8u(ts + base + 1 + my) ^ load8u(qd + base + 1 + my);
+				const u64 x = load8u(seq, so + (u32)(base + 1 + my)) ^ load8u(q, (u32)(dd + base + 1 + my));
 				m = x ? (i32)(__builtin_ctzll(x) >> 3) : 8;
 			}
 			const bool stop = (my < avail) && (m < 8 || my + 8 >= avail);
@@ -1234,8 +1231,8 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 				kk[s] = key_k(key[s]);
 				mk8[s] = (ql - d < vl ? ql - d : vl) - 1;
 				const bool go = key[s] != EMPTY_KEY && mk8[s] > kk[s];
-				const u64 gx = load8u(g.seq + (go ? vi[s].x + kk[s] + 1 : 0u));
-				const u64 qx = load8u(q + (go ? d + kk[s] + 1 : 0));
+				const u64 gx = load8u(g.seq, go ? vi[s].x + (u32)(kk[s] + 1) : 0u);
+				const u64 qx = load8u(q, go ? (u32)(d + kk[s] + 1) : 0u);
 				x8[s] = go ? gx ^ qx : 0;
 			}
 			bool more[S];
@@ -1722,7 +1719,7 @@ __global__ __launch_bounds__(WAVE *NW) __attribute__((amdgpu_waves_per_eu(5))) v
 	for (;;) {
 		if (b.tid() == 0) next_q = atomicAdd(&heads[bin], 1);
 		b.sync();
-		const i32 qi = next_q;
+		const i32 qi = __builtin_amdgcn_readfirstlane(next_q);  // uniform: the read's pointers stay scalar
 		b.sync();
 		if (qi >= q_n) break;
 		const i32 r = order ? order[q_lo + qi] : q_lo + qi;

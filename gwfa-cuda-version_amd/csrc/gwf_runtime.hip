@@ -488,7 +488,7 @@ int gwf_dreads_upload(int32_t n, const char *seqs, const int64_t *off, const int
 		dr->v0[i] = v0 ? v0[i] : 0;
 		if (dr->v0[i] < 0) { delete dr; set_err("gwf_dreads_upload: read %d has v0 %d", i, dr->v0[i]); return GWF_ERR_ARG; }
 		noff[i] = total;
-		total += ql[i] + 1;
+		total += (ql[i] + 8) & ~7;  // every read starts 8-byte aligned (the extension's loads need it)
 	}
 	std::vector<uint8_t> buf((size_t)total + kSeqPad, 0);
 	for (i32 i = 0; i < n; ++i) memcpy(&buf[noff[i]], seqs + off[i], ql[i]);
