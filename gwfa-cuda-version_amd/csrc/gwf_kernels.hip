@@ -44,6 +44,13 @@
 #define SORT_CHUNK 1024    // plain kernel: run length sorted in LDS
 #define EMPTY_KEY (~0ull)
 
+// Element types of LDS arrays.  Pointers to them are 32-bit LDS addresses,
+// so the kernels' many array pointers do not take 64-bit scalar register
+// pairs, and every access through them is a ds_ instruction.
+#define LDS_AS __attribute__((address_space(3)))
+typedef LDS_AS u64 lu64;
+typedef LDS_AS i32 li32;
+
 namespace GWF_KNS {
 
 // ------------------------------------------------------------ key helpers
@@ -132,9 +139,9 @@ __device__ __forceinline__ u64 lane_bcast64(u64 v, i32 l) {
 template <int NW>
 struct Blk {
 	static constexpr int NT = WAVE * NW;
-	i32 *red;   // LDS scratch, 2 NW ints
-	u64 *redu;  // LDS scratch, 2 NW u64
-	i32 *xch;   // LDS, 4 ints: results handed from one wave to the others
+	li32 *red;   // LDS scratch, 2 NW ints
+	lu64 *redu;  // LDS scratch, 2 NW u64
+	li32 *xch;   // LDS, 4 ints: results handed from one wave to the others
 	mutable i32 par = 0;  // slot of the next collective
 	__device__ __forceinline__ u32 tid() const { return threadIdx.x; }
 	__device__ __forceinline__ u32 wid() const { return NW == 1 ? 0u : threadIdx.x / WAVE; }
@@ -145,7 +152,7 @@ struct Blk {
 			*tot = c;
 			return 0;
 		} else {
-			i32 *r = red + par * NW;
+			li32 *r = red + par * NW;
 			par ^= 1;
 			if (lane_id() == 0) r[wid()] = c;
 			sync();
@@ -213,7 +220,7 @@ struct Blk {
 		if constexpr (NW == 1) {
 			return x;
 		} else {
-			i32 *r = red + par * NW;
+			li32 *r = red + par * NW;
 			par ^= 1;
 			if (lane_id() == 0) r[wid()] = x;
 			sync();
@@ -233,7 +240,7 @@ struct Blk {
 			*tot = lane_bcast64(wi, WAVE - 1);
 			return wi;
 		} else {
-			u64 *r = redu + par * NW;
+			lu64 *r = redu + par * NW;
 			par ^= 1;
 			if (lane_id() == WAVE - 1) r[wid()] = wi;
 			sync();
@@ -321,7 +328,8 @@ __device__ __forceinline__ u64 load8u(const uint8_t *base, u32 off) {
 }
 
 // first index in a[0..n) with a[i] >= x (lower) or a[i] > x (upper)
-__device__ __forceinline__ i32 lower_bound(const u64 *a, i32 n, u64 x) {
+template <class PA>
+__device__ __forceinline__ i32 lower_bound(PA a, i32 n, u64 x) {
 	i32 lo = 0, hi = n;
 	while (lo < hi) {
 		const i32 m = (lo + hi) >> 1;
@@ -330,7 +338,8 @@ __device__ __forceinline__ i32 lower_bound(const u64 *a, i32 n, u64 x) {
 	}
 	return lo;
 }
-__device__ __forceinline__ i32 upper_bound(const u64 *a, i32 n, u64 x) {
+template <class PA>
+__device__ __forceinline__ i32 upper_bound(PA a, i32 n, u64 x) {
 	i32 lo = 0, hi = n;
 	while (lo < hi) {
 		const i32 m = (lo + hi) >> 1;
@@ -442,15 +451,15 @@ __device__ __forceinline__ u64 wave_sort64(u64 x) {
 // into dst (left run first on ties).  Thread t writes outputs [t E, t E + E)
 // (E a power of two dividing 2 wdt, so a range never crosses a pair): one
 // merge-path search, then a sequential merge from the two run heads.
-template <int NW>
-__device__ __forceinline__ void merge_round(const u64 *src, u64 *dst, i32 n, i32 wdt, const Blk<NW> &b) {
+template <int NW, class PS, class PD>
+__device__ __forceinline__ void merge_round(PS src, PD dst, i32 n, i32 wdt, const Blk<NW> &b) {
 	constexpr i32 NT = Blk<NW>::NT;
 	i32 E = 1;
 	while (E * NT < n && E < 2 * wdt) E <<= 1;
 	for (i32 o0 = (i32)b.tid() * E; o0 < n; o0 += NT * E) {
 		const i32 l = o0 / (2 * wdt) * (2 * wdt);
 		const i32 mid = l + wdt < n ? l + wdt : n, r = l + 2 * wdt < n ? l + 2 * wdt : n;
-		const u64 *A = src + l, *C = src + mid;
+		const PS A = src + l, C = src + mid;
 		const i32 na = mid - l, nc = r - mid, dg = o0 - l;
 		i32 lo = dg - nc > 0 ? dg - nc : 0, hi = dg < na ? dg : na;
 		while (lo < hi) {
@@ -479,8 +488,8 @@ __device__ __forceinline__ void merge_round(const u64 *src, u64 *dst, i32 n, i32
 // barriers), then each key's final rank is found by binary search in the
 // other runs (one pass, when there are at most mw runs), or runs are merged
 // pairwise through dst/src (log2(runs) passes).  mw is a tuning parameter.
-template <int NW>
-__device__ __forceinline__ void sort_runs(u64 *src, i32 n, u64 *dst, i32 mw, const Blk<NW> &b) {
+template <int NW, class PT>
+__device__ __forceinline__ void sort_runs(PT src, i32 n, PT dst, i32 mw, const Blk<NW> &b) {
 	const i32 k = (n + WAVE - 1) / WAVE;
 	for (i32 c = (i32)b.wid(); c < k; c += NW) {
 		const i32 j = c * WAVE + (i32)lane_id();
@@ -509,11 +518,11 @@ __device__ __forceinline__ void sort_runs(u64 *src, i32 n, u64 *dst, i32 mw, con
 		b.sync();
 		return;
 	}
-	u64 *a = src, *c = dst;
+	PT a = src, c = dst;
 	for (i32 wdt = WAVE; wdt < n; wdt <<= 1) {
 		merge_round(a, c, n, wdt, b);
 		b.sync();
-		u64 *t = a;
+		PT t = a;
 		a = c;
 		c = t;
 	}
@@ -527,7 +536,7 @@ __device__ __forceinline__ void sort_runs(u64 *src, i32 n, u64 *dst, i32 mw, con
 // a power of two >= 128, elements of it), then merge-path rounds between p
 // and tmp (n elements).
 template <int NW>
-__device__ __forceinline__ void sort_hbm(u64 *p, i32 n, u64 *lds, i32 lc, u64 *tmp, const Blk<NW> &b) {
+__device__ __forceinline__ void sort_hbm(u64 *p, i32 n, lu64 *lds, i32 lc, u64 *tmp, const Blk<NW> &b) {
 	if (n <= 1) return;
 	const i32 cw = lc / 2;
 	for (i32 c0 = 0; c0 < n; c0 += cw) {
@@ -553,9 +562,9 @@ __device__ __forceinline__ void sort_hbm(u64 *p, i32 n, u64 *lds, i32 lc, u64 *t
 }
 
 // Merge two interval lists sorted by lo (a before c on ties) into olo/ohi.
-template <class B>
-__device__ __forceinline__ void merge_intervals2(const u64 *alo, const u64 *ahi, i32 na, const u64 *clo,
-                                                 const u64 *chi, i32 nc, u64 *olo, u64 *ohi, const B &b) {
+template <class B, class PA, class PC, class PO>
+__device__ __forceinline__ void merge_intervals2(PA alo, PA ahi, i32 na, PC clo, PC chi, i32 nc, PO olo, PO ohi,
+                                                 const B &b) {
 	for (i32 j = (i32)b.tid(); j < na; j += B::NT) {
 		const u64 x = alo[j];
 		const i32 p = j + lower_bound(clo, nc, x);
@@ -572,9 +581,9 @@ __device__ __forceinline__ void merge_intervals2(const u64 *alo, const u64 *ahi,
 // The same merge, partitioned into contiguous ranges of the output: one
 // merge-path search per thread, then a sequential merge (for lists in HBM,
 // where a binary search per element is a chain of dependent loads).
-template <int NW>
-__device__ __forceinline__ void merge_intervals_ranges(const u64 *alo, const u64 *ahi, i32 na, const u64 *clo,
-                                                       const u64 *chi, i32 nc, u64 *olo, u64 *ohi, const Blk<NW> &b) {
+template <int NW, class PA, class PC, class PO>
+__device__ __forceinline__ void merge_intervals_ranges(PA alo, PA ahi, i32 na, PC clo, PC chi, i32 nc, PO olo, PO ohi,
+                                                       const Blk<NW> &b) {
 	constexpr i32 NT = Blk<NW>::NT;
 	const i32 n = na + nc;
 	const i32 E = (n + NT - 1) / NT;
@@ -606,8 +615,8 @@ __device__ __forceinline__ void merge_intervals_ranges(const u64 *alo, const u64
 
 // Runs of consecutive ids in the sorted, duplicate-free T[0..n) as half-open
 // intervals [lo, hi); returns their count.
-template <class B>
-__device__ __forceinline__ i32 ti_runs(const u64 *T, i32 n, u64 *lo, u64 *hi, const B &b) {
+template <class B, class PT, class PO>
+__device__ __forceinline__ i32 ti_runs(PT T, i32 n, PO lo, PO hi, const B &b) {
 	i32 n_ti = 0;
 	for (i32 base = 0; base < n; base += B::NT) {
 		const i32 j = base + (i32)b.tid();
@@ -627,8 +636,8 @@ __device__ __forceinline__ i32 ti_runs(const u64 *T, i32 n, u64 *lo, u64 *hi, co
 // Coalesce overlapping or adjacent intervals of a list sorted by lo
 // (gwf_intv_merge_adj, gwf-ed.cu:266-296).  Returns the output count (the
 // caller guarantees room for n).
-template <class B>
-__device__ __forceinline__ i32 coalesce(const u64 *lo, const u64 *hi, i32 n, u64 *olo, u64 *ohi, const B &b) {
+template <class B, class PI, class PO>
+__device__ __forceinline__ i32 coalesce(PI lo, PI hi, i32 n, PO olo, PO ohi, const B &b) {
 	u64 carry = 0;  // max hi over previous chunks
 	i32 ng = 0;     // groups started so far
 	for (i32 base = 0; base < n; base += B::NT) {
@@ -654,7 +663,8 @@ __device__ __forceinline__ i32 coalesce(const u64 *lo, const u64 *hi, i32 n, u64
 	return ng;
 }
 
-__device__ __forceinline__ bool forbidden(const u64 *hlo, const u64 *hhi, i32 nh, u64 dg) {
+template <class PH>
+__device__ __forceinline__ bool forbidden(PH hlo, PH hhi, i32 nh, u64 dg) {
 	const i32 p = upper_bound(hlo, nh, dg) - 1;
 	return p >= 0 && dg < hhi[p];
 }
@@ -664,8 +674,8 @@ __device__ __forceinline__ bool forbidden(const u64 *hlo, const u64 *hhi, i32 nh
 typedef const __attribute__((address_space(3))) u64 *lds_cptr;
 
 // dst[0..n) = src[0..n), four loads in flight per thread (no barrier).
-template <class B>
-__device__ __forceinline__ void stage(u64 *dst, const u64 *src, i32 n, const B &b) {
+template <class B, class PD, class PS>
+__device__ __forceinline__ void stage(PD dst, PS src, i32 n, const B &b) {
 	constexpr i32 NT = B::NT;
 	for (i32 j0 = (i32)b.tid(); j0 < n; j0 += 4 * NT) {
 		u64 v[4];
@@ -686,10 +696,11 @@ __device__ __forceinline__ void stage(u64 *dst, const u64 *src, i32 n, const B &
 // first on ties) into x and returns whether it is kept (first of its
 // diagonal, outside the history intervals).  The cursors live in the caller;
 // an exhausted list's head is EMPTY_KEY, above every key; h is the first
-// history interval ending after the previous kept diagonal.
-template <typename QP, typename XP>
-__device__ __forceinline__ bool mf_next(QP Q, i32 n_s, XP X, i32 n_x, const u64 *Hl, const u64 *Hh, i32 nh, i32 &i,
-                                        i32 &jx, u64 &qh, u64 &xh, u64 &prev_dg, i32 &h, u64 &x) {
+// history interval ending after the previous kept diagonal, (hl, hh) its
+// bounds (EMPTY_KEY past the last one).
+template <typename QP, typename XP, typename HP>
+__device__ __forceinline__ bool mf_next(QP Q, i32 n_s, XP X, i32 n_x, HP Hl, HP Hh, i32 nh, i32 &i,
+                                        i32 &jx, u64 &qh, u64 &xh, u64 &prev_dg, i32 &h, u64 &hl, u64 &hh, u64 &x) {
 	const bool takeq = qh <= xh;
 	x = takeq ? qh : xh;
 	if (takeq) {
@@ -703,8 +714,12 @@ __device__ __forceinline__ bool mf_next(QP Q, i32 n_s, XP X, i32 n_x, const u64 
 	bool keep = dg != prev_dg;
 	prev_dg = dg;
 	if (keep) {
-		while (h < nh && Hh[h] <= dg) ++h;
-		keep = !(h < nh && Hl[h] <= dg);  // coalesced intervals: disjoint, sorted
+		while (hh <= dg) {
+			++h;
+			hl = h < nh ? Hl[h] : EMPTY_KEY;
+			hh = h < nh ? Hh[h] : EMPTY_KEY;
+		}
+		keep = hl > dg;  // coalesced intervals: disjoint, sorted
 	}
 	return keep;
 }
@@ -717,9 +732,9 @@ __device__ __forceinline__ bool mf_next(QP Q, i32 n_s, XP X, i32 n_x, const u64 
 // diagonal and tests the history (coalesced intervals Hl/Hh, nh of them)
 // with a pointer that only moves forward.  Kept states go to P: straight from
 // registers, in chunks of 8 steps (see below).  Returns their count.
-// QP and XP are lds_cptr or const u64 * (where each list lives).
-template <int NW, typename QP, typename XP>
-__device__ __forceinline__ i32 merge_filter(QP Q, i32 n_s, XP X, i32 n_x, const u64 *Hl, const u64 *Hh, i32 nh, u64 *M,
+// QP, XP and HP are lds_cptr or const u64 * (where each list lives).
+template <int NW, typename QP, typename XP, typename HP>
+__device__ __forceinline__ i32 merge_filter(QP Q, i32 n_s, XP X, i32 n_x, HP Hl, HP Hh, i32 nh, u64 *M,
                                             u64 *P, const Blk<NW> &b) {
 	constexpr i32 NT = Blk<NW>::NT;
 	constexpr i32 CH = 8;
@@ -728,7 +743,7 @@ __device__ __forceinline__ i32 merge_filter(QP Q, i32 n_s, XP X, i32 n_x, const 
 	const i32 o0 = (i32)b.tid() * E;
 	const i32 o1 = o0 + E < n_m ? o0 + E : n_m;
 	i32 i = 0, jx = 0, h = 0;
-	u64 prev_dg = ~0ull, qh = EMPTY_KEY, xh = EMPTY_KEY;
+	u64 prev_dg = ~0ull, qh = EMPTY_KEY, xh = EMPTY_KEY, hl = EMPTY_KEY, hh = EMPTY_KEY;
 	if (o0 < n_m) {
 		i32 lo = o0 - n_x > 0 ? o0 - n_x : 0, up = o0 < n_s ? o0 : n_s;
 		while (lo < up) {
@@ -750,6 +765,8 @@ __device__ __forceinline__ i32 merge_filter(QP Q, i32 n_s, XP X, i32 n_x, const 
 			if (Hh[mid] <= dg0) h = mid + 1;
 			else c = mid;
 		}
+		hl = h < nh ? Hl[h] : EMPTY_KEY;
+		hh = h < nh ? Hh[h] : EMPTY_KEY;
 	}
 	// Chunks of CH merge steps in registers.  The last chunk's kept states
 	// stay in registers until the scan has given their place in P; earlier
@@ -771,7 +788,7 @@ __device__ __forceinline__ i32 merge_filter(QP Q, i32 n_s, XP X, i32 n_x, const 
 #pragma unroll
 		for (i32 u = 0; u < CH; ++u) {
 			u64 x = 0;
-			if (o + u < o1 && mf_next(Q, n_s, X, n_x, Hl, Hh, nh, i, jx, qh, xh, prev_dg, h, x)) km |= 1u << u;
+			if (o + u < o1 && mf_next(Q, n_s, X, n_x, Hl, Hh, nh, i, jx, qh, xh, prev_dg, h, hl, hh, x)) km |= 1u << u;
 			kv[u] = x;
 		}
 	}
@@ -800,11 +817,13 @@ __device__ __forceinline__ u32 vis_hash(u64 x) {
 }
 // true if key was absent (and is now present); a full probe window reports
 // "absent", which only costs duplicate work.
-__device__ __forceinline__ bool vis_insert(u64 *vis, u64 key) {
+__device__ __forceinline__ bool vis_insert(lu64 *vis, u64 key) {
 	u32 h = vis_hash(key) & (VIS_SLOTS - 1);
 	for (int p = 0; p < 16; ++p) {
-		const u64 old = atomicCAS((unsigned long long *)&vis[h], (unsigned long long)EMPTY_KEY, (unsigned long long)key);
-		if (old == EMPTY_KEY) return true;
+		u64 old = EMPTY_KEY;
+		if (__hip_atomic_compare_exchange_strong(&vis[h], &old, key, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+		                                         __HIP_MEMORY_SCOPE_WORKGROUP))
+			return true;
 		if (old == key) return false;
 		h = (h + 1) & (VIS_SLOTS - 1);
 	}
@@ -823,7 +842,7 @@ __device__ __forceinline__ i32 vs_slot(const BF &w, u64 K, bool insert) {
 		if (old == K) return (i32)h;
 		if (old == EMPTY_KEY) {
 			if (!insert) return -1;
-			atomicAdd(w.vfill, 1);
+			__hip_atomic_fetch_add(w.vfill, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 			return (i32)h;
 		}
 		h = (h + 1) & w.vmask;
@@ -832,10 +851,12 @@ __device__ __forceinline__ i32 vs_slot(const BF &w, u64 K, bool insert) {
 }
 
 // ------------------------------------------------------------ working arrays
-struct Bufs {
-	u64 *P, *Q, *X, *F0, *F1, *T;
-	u64 *H0lo, *H0hi, *H1lo, *H1hi;  // forbidden-interval history, double buffered
-	u64 *sort_lds;    // HBM mode: LDS scratch for sorting runs and staging
+// One read's arrays: in LDS (T = lu64) or in HBM (T = u64).
+template <class E>
+struct BufsT {
+	E *P, *Q, *X, *F0, *F1, *T;
+	E *H0lo, *H0hi, *H1lo, *H1hi;  // forbidden-interval history, double buffered
+	lu64 *sort_lds;   // HBM mode: LDS scratch for sorting runs and staging
 	i32 sort_cap;     // its capacity for bitonic runs (power of two)
 	i32 sort_n;       // its size in elements
 	i32 mw;           // LDS sort: runs of 64 merged in one rank pass when at most mw of them
@@ -844,9 +865,11 @@ struct Bufs {
 	// none): keys, and per key the stamp of the candidate that owns it
 	u64 *VK, *VS;
 	u32 vmask;        // slots - 1
-	i32 *vfill;       // LDS: slots taken
+	li32 *vfill;      // LDS: slots taken
 	Caps c;
 };
+typedef BufsT<lu64> LBufs;
+typedef BufsT<u64> HBufs;
 
 __host__ __device__ inline size_t ws_elems(const Caps &c) {
 	return 2 * (size_t)c.m + c.x + 2 * (size_t)c.f + c.t + 4 * (size_t)c.h + (size_t)(c.x > c.t ? c.x : c.t) + 8 +
@@ -858,8 +881,8 @@ __host__ __device__ inline size_t lds_elems(const Caps &c) {
 	return 2 * (size_t)c.m + c.x + 2 * (size_t)c.f + c.t + 4 * (size_t)c.h;
 }
 
-__device__ inline Bufs carve(u64 *p, const Caps &c, bool with_tmp) {
-	Bufs w;
+template <class E>
+__device__ inline void carve_arrays(BufsT<E> &w, E *p, const Caps &c) {
 	w.c = c;
 	w.P = p; p += c.m;
 	w.Q = p; p += c.m;
@@ -871,16 +894,32 @@ __device__ inline Bufs carve(u64 *p, const Caps &c, bool with_tmp) {
 	w.H0hi = p; p += c.h;
 	w.H1lo = p; p += c.h;
 	w.H1hi = p; p += c.h;
-	w.sort_tmp = with_tmp ? p : nullptr;
-	p += (c.x > c.t ? c.x : c.t) + 8;
-	w.VK = with_tmp && c.v ? p : nullptr;
-	w.VS = p + c.v;
-	w.vmask = c.v ? (u32)c.v - 1 : 0;
+	w.sort_tmp = nullptr;
+	w.VK = w.VS = nullptr;
+	w.vmask = 0;
 	w.vfill = nullptr;
 	w.sort_lds = nullptr;
 	w.sort_n = 0;
 	w.mw = 8;
 	w.sort_cap = 0;
+}
+// LDS-resident arrays (no merge temp, no visited set of their own)
+__device__ inline LBufs carve_lds(lu64 *p, const Caps &c) {
+	LBufs w;
+	carve_arrays(w, p, c);
+	return w;
+}
+// An HBM workspace: the arrays, the merge temp and (mixed in-overlaps) the
+// read's visited set
+__device__ inline HBufs carve(u64 *p, const Caps &c) {
+	HBufs w;
+	carve_arrays(w, p, c);
+	p += lds_elems(c);
+	w.sort_tmp = p;
+	p += (c.x > c.t ? c.x : c.t) + 8;
+	w.VK = c.v ? p : nullptr;
+	w.VS = p + c.v;
+	w.vmask = c.v ? (u32)c.v - 1 : 0;
 	return w;
 }
 
@@ -925,12 +964,12 @@ enum { STEP_NEXT = 0, STEP_FOUND = 1, STEP_EMPTY = 2, STEP_SPILL = 3 };
 // stays for the filter, and to Hlo/Hhi in HBM for the next step.  Needs
 // 2 n_tt + 2 n_h + 4 (n_h + n_tt) <= sort_n.  Returns its interval count.
 template <class B>
-__device__ __forceinline__ i32 hist_lds(const Bufs &w, const B &b, const u64 *Hc_lo, const u64 *Hc_hi, i32 n_h, i32 n_tt,
+__device__ __forceinline__ i32 hist_lds(const HBufs &w, const B &b, const u64 *Hc_lo, const u64 *Hc_hi, i32 n_h, i32 n_tt,
                                         u64 *Hlo, u64 *Hhi) {
 	const i32 cap_h = n_h + n_tt;
-	u64 *l = w.sort_lds;
-	u64 *hc = l + 2 * n_tt, *Clo = hc + 2 * n_h, *Chi = Clo + cap_h;
-	u64 *Olo = l + w.sort_n - 2 * cap_h, *Ohi = Olo + cap_h;
+	lu64 *l = w.sort_lds;
+	lu64 *hc = l + 2 * n_tt, *Clo = hc + 2 * n_h, *Chi = Clo + cap_h;
+	lu64 *Olo = l + w.sort_n - 2 * cap_h, *Ohi = Olo + cap_h;
 	stage(hc, Hc_lo, n_h, b);
 	stage(hc + n_h, Hc_hi, n_h, b);
 	const i32 n_ti = ti_runs(w.T, n_tt, l, l + n_tt, b);
@@ -951,8 +990,8 @@ __device__ __forceinline__ i32 hist_lds(const Bufs &w, const B &b, const u64 *Hc
 // staying; pass 2 keeps the candidates that own their key and writes the
 // next level's entries state by state, arc by arc.  Returns true when the
 // set is full (the caller reports a spill; the read is re-run larger).
-template <class B>
-__device__ __forceinline__ bool closure_mixed(const DGraph &g, const uint8_t *q, i32 ql, const Bufs &w, const B &b,
+template <class B, class E>
+__device__ __forceinline__ bool closure_mixed(const DGraph &g, const uint8_t *q, i32 ql, const BufsT<E> &w, const B &b,
                                               i32 v1, i32 n_f, bool closure_origin, u64 &lvl, i32 &n_na, i32 &n_x,
                                               i32 &n_t) {
 	constexpr i32 NT = B::NT;
@@ -1062,8 +1101,8 @@ __device__ __forceinline__ bool closure_mixed(const DGraph &g, const uint8_t *q,
 // position) once.  Appends successors to X and (v1 >= 0) exits to T;
 // returns STEP_NEXT, STEP_FOUND or STEP_SPILL.  B: the whole workgroup
 // (Blk) or one wave (WaveB).
-template <bool MIXED, class B>
-__device__ __forceinline__ int closure(const DGraph &g, const uint8_t *q, i32 ql, const Bufs &w, const B &b, u64 *vis,
+template <bool MIXED, class B, class E>
+__device__ __forceinline__ int closure(const DGraph &g, const uint8_t *q, i32 ql, const BufsT<E> &w, const B &b, lu64 *vis,
                                        i32 v1, i32 n_f, i32 &n_x, i32 &n_t, int64_t &chars, u64 &lvl) {
 	constexpr i32 NT = B::NT;
 	const i32 tid = (i32)b.tid();
@@ -1093,28 +1132,58 @@ __device__ __forceinline__ int closure(const DGraph &g, const uint8_t *q, i32 ql
 			const i32 deg = act ? (i32)vi.w : 0;
 			const i32 maxdeg = b.max(deg);
 			i32 n_ext = 0;
-			for (i32 e = 0; e < maxdeg; ++e) {
-				const bool has = e < deg;
-				const uint4 ar = has ? g.arc[vi.z + e] : make_uint4(0, 0, 0, 0);
-				const u32 wv = ar.x;
-				const i32 ol = (i32)(ar.y & 0xFFFFFFu);
-				const bool m = has && !qe && (uint8_t)(ar.y >> 24) == c;
-				n_ext += m;
-				const bool absent = has && !qe && vis_insert(vis, (u64)wv << 32 | (u32)(i + 1));
-				const bool pa = absent && m;
-				const bool px = absent && !m;
-				const bool pq = has && qe;
+			// arcs in groups of AG: the group's arc records are loaded together
+			// (from a valid address when the lane has no arc there: loads under
+			// a branch are waited for one after the other), its claims made, and
+			// its successors placed with one scan
+			constexpr i32 AG = 4;
+			for (i32 e0 = 0; e0 < maxdeg; e0 += AG) {
+				uint2 ar[AG];
+#pragma unroll
+				for (i32 u = 0; u < AG; ++u) {
+					const bool has = e0 + u < deg;
+					const uint2 t = *(const uint2 *)&g.arc[has ? vi.z + (u32)(e0 + u) : 0u];
+					ar[u] = has ? t : make_uint2(0, 0);
+				}
+				u32 pa_m = 0, px_m = 0, pq_m = 0;
+				i32 ca = 0, cx = 0;
+#pragma unroll
+				for (i32 u = 0; u < AG; ++u) {
+					const bool has = e0 + u < deg;
+					const bool m = has && !qe && (uint8_t)(ar[u].y >> 24) == c;
+					n_ext += m;
+					const bool absent = has && !qe && vis_insert(vis, (u64)ar[u].x << 32 | (u32)(i + 1));
+					pa_m |= (absent && m ? 1u : 0u) << u;
+					px_m |= (absent && !m ? 1u : 0u) << u;
+					pq_m |= (has && qe ? 1u : 0u) << u;
+				}
+				ca = __popc(pa_m);
+				cx = 2 * __popc(px_m) + __popc(pq_m);
 				i32 sx, ta, tx;
-				const i32 sa = n_na + b.scan2(pa ? 1 : 0, px ? 2 : pq ? 1 : 0, &sx, &ta, &tx);
+				i32 sa = n_na + b.scan2(ca, cx, &sx, &ta, &tx);
 				sx += n_x;
 				n_na += ta;
 				n_x += tx;
-				if (pa && sa < cap.f) w.F1[sa] = mk(wv, i + 1 - ol, ol);
-				if (px && sx + 1 < cap.x) {
-					w.X[sx] = mk(wv, i - ol, ol);
-					w.X[sx + 1] = mk(wv, i + 1 - ol, ol);
+#pragma unroll
+				for (i32 u = 0; u < AG; ++u) {
+					const u32 wv = ar[u].x;
+					const i32 ol = (i32)(ar[u].y & 0xFFFFFFu);
+					if ((pa_m >> u) & 1) {
+						if (sa < cap.f) w.F1[sa] = mk(wv, i + 1 - ol, ol);
+						++sa;
+					}
+					if ((px_m >> u) & 1) {
+						if (sx + 1 < cap.x) {
+							w.X[sx] = mk(wv, i - ol, ol);
+							w.X[sx + 1] = mk(wv, i + 1 - ol, ol);
+						}
+						sx += 2;
+					}
+					if ((pq_m >> u) & 1) {
+						if (sx < cap.x) w.X[sx] = mk(wv, i - ol, ol);
+						++sx;
+					}
 				}
-				if (pq && sx < cap.x) w.X[sx] = mk(wv, i - ol, ol);
 			}
 			const bool pi = rec && !qe && (deg == 0 || n_ext != deg);
 			// with v1 >= 0 the exits of states that entered their vertex in
@@ -1175,9 +1244,39 @@ __device__ __forceinline__ int closure(const DGraph &g, const uint8_t *q, i32 ql
 // updated history in H[hs ^ 1].  On STEP_SPILL nothing the next attempt needs
 // was modified (P holds the same states, extended in place, which is
 // idempotent).  HBM selects how sorts are staged.
+template <bool HBM> struct ElemOf { typedef lu64 type; };
+template <> struct ElemOf<true> { typedef u64 type; };
+
+// P6 for arrays that stay where they are: merge the sorted X into Q in place
+// (merge path, right to left; Q first on ties).
+template <int NW, class PQ, class PX>
+__device__ __forceinline__ void merge_in_place(PQ Q, i32 n_s, PX Xs, i32 n_x, const Blk<NW> &b) {
+	constexpr i32 NT = Blk<NW>::NT;
+	const i32 tid = (i32)b.tid();
+	for (i32 hi = n_s + n_x; hi > 0; hi -= NT) {
+		const i32 p = hi - NT + tid;
+		u64 val = 0;
+		if (p >= 0) {
+			i32 lo = p - n_x > 0 ? p - n_x : 0, up = p < n_s ? p : n_s;
+			while (lo < up) {
+				const i32 mid = (lo + up) >> 1;
+				if (Q[mid] <= Xs[p - 1 - mid]) lo = mid + 1;
+				else up = mid;
+			}
+			const i32 i = lo, jx = p - lo;
+			val = (i < n_s && (jx >= n_x || Q[i] <= Xs[jx])) ? Q[i] : Xs[jx];
+		}
+		b.sync();
+		if (p >= 0) Q[p] = val;
+		b.sync();
+	}
+}
+
 template <int NW, bool HBM, bool MIXED>
-__device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 ql, const Bufs &w, const Blk<NW> &b,
-                                        u64 *vis, i32 v1, i32 &n_a, i32 &n_h, i32 &hs, int64_t &chars, u64 &lvl) {
+__device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 ql,
+                                        const BufsT<typename ElemOf<HBM>::type> &w, const Blk<NW> &b, lu64 *vis,
+                                        i32 v1, i32 &n_a, i32 &n_h, i32 &hs, int64_t &chars, u64 &lvl) {
+	typedef typename ElemOf<HBM>::type E;
 	constexpr i32 NT = Blk<NW>::NT;
 	const i32 tid = (i32)b.tid();
 	const Caps &cap = w.c;
@@ -1198,7 +1297,7 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 	const i32 R = (n_a + RS - 1) / RS;
 	const i32 wv = (i32)b.wid();
 	const u32 lane = lane_id();
-	u64 *edgeF = vis, *edgeL = vis + 3 * NW;  // [3][NW]: first / last state of each wave's range
+	lu64 *edgeF = vis, *edgeL = vis + 3 * NW;  // [3][NW]: first / last state of each wave's range
 	bool found = false, ovf = false;
 	i32 n_f = 0, n_s = 0, n_t = 0;
 	u64 pk[S];   // the previous round's extended states (EMPTY_KEY past the end)
@@ -1344,11 +1443,12 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 	// (with four waves both stay block-wide: measured faster).  A spill
 	// found by the closure then ends the read's attempt (HBM capacities are
 	// re-sized by the host), so P5 having run changes nothing.
-	u64 *Hcur_lo = hs ? w.H1lo : w.H0lo, *Hcur_hi = hs ? w.H1hi : w.H0hi;
-	u64 *Hlo = hs ? w.H0lo : w.H1lo, *Hhi = hs ? w.H0hi : w.H1hi;
+	E *Hcur_lo = hs ? w.H1lo : w.H0lo, *Hcur_hi = hs ? w.H1hi : w.H0hi;
+	E *Hlo = hs ? w.H0lo : w.H1lo, *Hhi = hs ? w.H0hi : w.H1hi;
 	i32 n_x = 0, nh2 = 0;
-	i32 hist_res = 0;                    // HBM mode: LDS elements reserved at the top of the scratch
-	const u64 *Hl_f = Hlo, *Hh_f = Hhi;  // where the filter reads the new history
+	i32 hist_res = 0;  // HBM mode: LDS elements reserved at the top of the scratch
+	// HBM mode: the new history's copy in the LDS scratch (null: only in HBM)
+	const lu64 *Hl_l = nullptr, *Hh_l = nullptr;
 	i32 q_res = 0;                       // HBM mode: Q staged in the LDS scratch just below the history
 	bool p5_done = false;
 	if constexpr (HBM && NW == 2) {
@@ -1371,7 +1471,7 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 			if (rc != STEP_NEXT) return rc;
 			hist_res = 2 * cap_h;
 			q_res = n_s <= w.sort_n - 2 * cap_h ? n_s : 0;
-			Hl_f = w.sort_lds + w.sort_n - 2 * cap_h, Hh_f = Hl_f + cap_h;
+			Hl_l = w.sort_lds + w.sort_n - 2 * cap_h, Hh_l = Hl_l + cap_h;
 			p5_done = true;
 		}
 	}
@@ -1388,7 +1488,7 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 	// ---- exits recorded by the closure (v1 >= 0 only): sort T, drop repeats
 	i32 n_tt = n_t1;
 	if (n_t > n_t1) {
-		const u64 *S;
+		const E *S;
 		if constexpr (HBM) {
 			sort_hbm(w.T, n_t, w.sort_lds, w.sort_cap, w.sort_tmp, b);
 			S = w.T;
@@ -1396,7 +1496,7 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 			sort_runs(w.T, n_t, w.P, w.mw, b);
 			S = w.P;
 		}
-		u64 *D = HBM ? w.P : w.T;
+		E *D = HBM ? w.P : w.T;
 		i32 nu = 0;
 		for (i32 base = 0; base < n_t; base += NT) {
 			const i32 j = base + tid;
@@ -1416,26 +1516,29 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 
 	// ---- P5: history H[hs ^ 1] = coalesce(H[hs] u runs of this step's exits)
 	const i32 cap_h = n_h + n_tt;  // bound on the merged interval count
-	if (p5_done) {
-		// (done beside the closure)
-	} else if (HBM && 2 * n_tt + 2 * n_h + 4 * cap_h <= w.sort_n) {
-		nh2 = hist_lds(w, b, Hcur_lo, Hcur_hi, n_h, n_tt, Hlo, Hhi);
-		hist_res = 2 * cap_h;
-		Hl_f = w.sort_lds + w.sort_n - 2 * cap_h, Hh_f = Hl_f + cap_h;
-	} else {
-		u64 *TIlo = w.P, *TIhi = w.P + n_tt;
+	if constexpr (HBM) {
+		if (!p5_done && 2 * n_tt + 2 * n_h + 4 * cap_h <= w.sort_n) {
+			nh2 = hist_lds(w, b, Hcur_lo, Hcur_hi, n_h, n_tt, Hlo, Hhi);
+			hist_res = 2 * cap_h;
+			Hl_l = w.sort_lds + w.sort_n - 2 * cap_h, Hh_l = Hl_l + cap_h;
+			p5_done = true;
+		}
+	}
+	if (!p5_done) {
+		E *TIlo = w.P, *TIhi = w.P + n_tt;
 		const i32 n_ti = ti_runs(w.T, n_tt, TIlo, TIhi, b);
 		b.sync();
-		u64 *Clo = w.P + 2 * n_tt, *Chi = Clo + (n_h + n_ti);
+		E *Clo = w.P + 2 * n_tt, *Chi = Clo + (n_h + n_ti);
 		if constexpr (HBM) merge_intervals_ranges(Hcur_lo, Hcur_hi, n_h, TIlo, TIhi, n_ti, Clo, Chi, b);
 		else merge_intervals2(Hcur_lo, Hcur_hi, n_h, TIlo, TIhi, n_ti, Clo, Chi, b);
 		nh2 = coalesce(Clo, Chi, n_h + n_ti, Hlo, Hhi, b);
 	}
 
-	// ---- P4: sort the closure successors into P
-	const u64 *Xs;
-	i32 lds_used = 0;     // HBM mode: elements of the LDS scratch holding the sorted X
-	bool x_lds = false;  // HBM mode: the sorted X is in the LDS scratch
+	// ---- P4: sort the closure successors: in LDS (Xl), or in HBM (Xg, HBM mode)
+	const lu64 *Xl = nullptr;
+	const u64 *Xg = nullptr;
+	i32 lds_used = 0;    // HBM mode: elements of the LDS scratch holding the sorted X
+	bool x_lds = !HBM;   // the sorted X is in LDS
 	if constexpr (HBM) {
 		const i32 free_n = w.sort_n - hist_res - q_res;  // LDS scratch below the staged history and Q
 		i32 free_cap = 128;
@@ -1444,7 +1547,7 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 			stage(w.sort_lds, w.X, n_x, b);
 			b.sync();
 			sort_runs(w.sort_lds, n_x, w.sort_lds + n_x, w.mw, b);
-			Xs = w.sort_lds + n_x;
+			Xl = w.sort_lds + n_x;
 			lds_used = 2 * n_x;
 			x_lds = true;
 		} else if ((hist_res || q_res) && free_cap >= 256 && free_cap <= free_n) {
@@ -1452,65 +1555,61 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 			// history and Q staged above it stay there for the merge (which
 			// then takes most of its inputs from LDS)
 			sort_hbm(w.X, n_x, w.sort_lds, free_cap, w.sort_tmp, b);
-			Xs = w.X;
+			Xg = w.X;
 		} else {
-			if (hist_res) {  // the sort takes the whole scratch: the filter reads the history from HBM
+			if (hist_res) {  // the sort takes the whole scratch: the history is staged again later
 				b.sync();
 				hist_res = 0, q_res = 0;
-				Hl_f = Hlo, Hh_f = Hhi;
+				Hl_l = Hh_l = nullptr;
 			}
 			sort_hbm(w.X, n_x, w.sort_lds, w.sort_cap, w.sort_tmp, b);
-			Xs = w.X;
+			Xg = w.X;
 		}
 	} else {
 		sort_runs(w.X, n_x, w.P, w.mw, b);
-		Xs = w.P;
+		Xl = w.P;
 	}
 
 	const i32 n_m = n_s + n_x;
 	i32 n_new = 0;
-	if (HBM && n_m <= 2 * cap.f) {
-		// ---- P6+P7 fused (arrays in HBM), from LDS copies where they fit:
-		// the sorted X is there already; the new history and Q follow it.
-		u64 *M = w.F0;  // F0 and F1 are adjacent: 2 cap.f >= n_m
-		const u64 *Hl = Hl_f, *Hh = Hh_f;
+	bool fused = false;
+	if constexpr (HBM) {
 		i32 used = lds_used;
 		const i32 avail = w.sort_n - hist_res - q_res;
-		if (!hist_res && used + 2 * nh2 <= avail) {
-			u64 *hl = w.sort_lds + used, *hh = hl + nh2;
+		if (!Hl_l && n_m <= 2 * cap.f && used + 2 * nh2 <= avail) {
+			lu64 *hl = w.sort_lds + used, *hh = hl + nh2;
 			stage(hl, Hlo, nh2, b);
 			stage(hh, Hhi, nh2, b);
-			Hl = hl, Hh = hh;
+			Hl_l = hl, Hh_l = hh;
 			used += 2 * nh2;
 		}
-		const bool q_lds = q_res || used + n_s <= avail;
-		u64 *Ql = q_res ? w.sort_lds + w.sort_n - hist_res - q_res : w.sort_lds + used;
-		if (q_lds && !q_res) stage(Ql, w.Q, n_s, b);
-		b.sync();
-		if (q_lds && x_lds) n_new = merge_filter<NW>((lds_cptr)Ql, n_s, (lds_cptr)Xs, n_x, Hl, Hh, nh2, M, w.P, b);
-		else if (q_lds) n_new = merge_filter<NW>((lds_cptr)Ql, n_s, Xs, n_x, Hl, Hh, nh2, M, w.P, b);
-		else if (x_lds) n_new = merge_filter<NW>((const u64 *)w.Q, n_s, (lds_cptr)Xs, n_x, Hl, Hh, nh2, M, w.P, b);
-		else n_new = merge_filter<NW>((const u64 *)w.Q, n_s, Xs, n_x, Hl, Hh, nh2, M, w.P, b);
-	} else {
-		// ---- P6: merge the sorted X into Q in place (merge path, right to left; Q first on ties)
-		if (n_x > 0) {
-			for (i32 hi = n_m; hi > 0; hi -= NT) {
-				const i32 p = hi - NT + tid;
-				u64 val = 0;
-				if (p >= 0) {
-					i32 lo = p - n_x > 0 ? p - n_x : 0, up = p < n_s ? p : n_s;
-					while (lo < up) {
-						const i32 mid = (lo + up) >> 1;
-						if (w.Q[mid] <= Xs[p - 1 - mid]) lo = mid + 1;
-						else up = mid;
-					}
-					const i32 i = lo, jx = p - lo;
-					val = (i < n_s && (jx >= n_x || w.Q[i] <= Xs[jx])) ? w.Q[i] : Xs[jx];
-				}
-				b.sync();
-				if (p >= 0) w.Q[p] = val;
-				b.sync();
+		if (n_m <= 2 * cap.f) {
+			// ---- P6+P7 fused (arrays in HBM), from LDS copies where they fit:
+			// the sorted X and the new history are there; Q follows them.
+			fused = true;
+			u64 *M = w.F0;  // F0 and F1 are adjacent: 2 cap.f >= n_m
+			const bool q_lds = q_res || used + n_s <= avail;
+			lu64 *Ql = q_res ? w.sort_lds + w.sort_n - hist_res - q_res : w.sort_lds + used;
+			if (q_lds && !q_res) stage(Ql, w.Q, n_s, b);
+			b.sync();
+			if (Hl_l) {
+				const lu64 *Hl = Hl_l, *Hh = Hh_l;
+				if (q_lds && x_lds) n_new = merge_filter<NW>((lds_cptr)Ql, n_s, Xl, n_x, Hl, Hh, nh2, M, w.P, b);
+				else if (q_lds) n_new = merge_filter<NW>((lds_cptr)Ql, n_s, Xg, n_x, Hl, Hh, nh2, M, w.P, b);
+				else if (x_lds) n_new = merge_filter<NW>((const u64 *)w.Q, n_s, Xl, n_x, Hl, Hh, nh2, M, w.P, b);
+				else n_new = merge_filter<NW>((const u64 *)w.Q, n_s, Xg, n_x, Hl, Hh, nh2, M, w.P, b);
+			} else {  // the history did not fit in LDS: every list through generic pointers
+				const u64 *Qg = q_lds ? (const u64 *)Ql : (const u64 *)w.Q;
+				const u64 *Xa = x_lds ? (const u64 *)Xl : Xg;
+				n_new = merge_filter<NW>(Qg, n_s, Xa, n_x, (const u64 *)Hlo, (const u64 *)Hhi, nh2, M, w.P, b);
 			}
+		}
+	}
+	if (!fused) {
+		// ---- P6: merge the sorted X into Q in place
+		if (n_x > 0) {
+			if (x_lds) merge_in_place(w.Q, n_s, Xl, n_x, b);
+			else merge_in_place(w.Q, n_s, Xg, n_x, b);
 		}
 
 		// ---- P7: furthest state per diagonal, outside the forbidden intervals
@@ -1534,9 +1633,9 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 
 // Copy the step input (wavefront + history) from LDS to HBM buffers.
 template <int NW>
-__device__ __forceinline__ void spill_state(const Bufs &from, const Bufs &to, i32 n_a, i32 n_h, i32 hs, const Blk<NW> &b) {
+__device__ __forceinline__ void spill_state(const LBufs &from, const HBufs &to, i32 n_a, i32 n_h, i32 hs, const Blk<NW> &b) {
 	for (i32 t = (i32)b.tid(); t < n_a; t += Blk<NW>::NT) to.P[t] = from.P[t];
-	const u64 *flo = hs ? from.H1lo : from.H0lo, *fhi = hs ? from.H1hi : from.H0hi;
+	const lu64 *flo = hs ? from.H1lo : from.H0lo, *fhi = hs ? from.H1hi : from.H0hi;
 	u64 *tlo = hs ? to.H1lo : to.H0lo, *thi = hs ? to.H1hi : to.H0hi;
 	for (i32 t = (i32)b.tid(); t < n_h; t += Blk<NW>::NT) {
 		tlo[t] = flo[t];
@@ -1562,8 +1661,8 @@ __device__ __forceinline__ void beat_put(GwfBeat *bt, i32 rid, i32 s, i32 n_a, i
 // Align one read: LDS-resident while it fits (USE_LDS), then in HBM.  beat
 // (may be null) receives a progress record every GWF_BEAT_EVERY steps.
 template <int NW, bool USE_LDS, bool MIXED>
-__device__ __forceinline__ ReadOut align_read(const DGraph &g, const uint8_t *q, i32 ql, u32 v0, i32 v1, const Bufs &lds,
-                              const Bufs &hbm, const Blk<NW> &b, u64 *vis, GwfBeat *beat, i32 rid) {
+__device__ __forceinline__ ReadOut align_read(const DGraph &g, const uint8_t *q, i32 ql, u32 v0, i32 v1, const LBufs &lds,
+                              const HBufs &hbm, const Blk<NW> &b, lu64 *vis, GwfBeat *beat, i32 rid) {
 	ReadOut o;
 	o.ed = -1, o.status = GWF_ST_OK, o.steps = 0, o.states = 0, o.chars = 0, o.spill_step = -1;
 	i32 n_a = 1, n_h = 0, hs = 0;
@@ -1575,7 +1674,10 @@ __device__ __forceinline__ ReadOut align_read(const DGraph &g, const uint8_t *q,
 		if (b.tid() == 0) *hbm.vfill = 0;
 		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
 	}
-	if (b.tid() == 0) (in_lds ? lds.P : hbm.P)[0] = mk(v0, 0, -1);
+	if (b.tid() == 0) {
+		if (in_lds) lds.P[0] = mk(v0, 0, -1);
+		else hbm.P[0] = mk(v0, 0, -1);
+	}
 	b.sync();
 	for (i32 s = 0;; ++s) {
 		o.steps += 1;
@@ -1652,20 +1754,22 @@ __global__ __launch_bounds__(WAVE) void gwf_plain_kernel(DGraph g, DReads rd, co
 	const i32 blk = blockIdx.x;
 	if (blk >= n) return;
 	const i32 r = read_idx ? read_idx[blk] : base + blk;
-	Blk<1> b{red, redu, xch};
+	Blk<1> b{(li32 *)red, (lu64 *)redu, (li32 *)xch};
 	const i32 a = ws_off ? ws_a[blk] : a_uniform;
 	Caps caps;
 	caps.m = 2 * a;
 	caps.x = caps.f = caps.t = caps.h = a;
 	caps.v = gwf_vis_slots(a, MIXED);
-	Bufs w = carve(scratch + (ws_off ? ws_off[blk] : (u64)blk * plain_stride(caps)), caps, true);
+	HBufs w = carve(scratch + (ws_off ? ws_off[blk] : (u64)blk * plain_stride(caps)), caps);
 	__shared__ i32 vfill;
-	w.vfill = &vfill;
-	w.sort_lds = lds_sort;
+	w.vfill = (li32 *)&vfill;
+	w.sort_lds = (lu64 *)lds_sort;
 	w.sort_cap = SORT_CHUNK;
 	w.sort_n = SORT_CHUNK;
 	const u64 t0 = wall_clock64();
-	const ReadOut o = align_read<1, false, MIXED>(g, rd.seq + rd.off[r], rd.ql[r], (u32)rd.v0[r], rd.v1, w, w, b, vis, nullptr, r);
+	const LBufs none{};  // (no LDS-resident arrays)
+	const ReadOut o = align_read<1, false, MIXED>(g, rd.seq + rd.off[r], rd.ql[r], (u32)rd.v0[r], rd.v1, none, w, b,
+	                                              (lu64 *)vis, nullptr, r);
 	write_out<1>(o, r, t0, out_ed, out_status, out_counts, b);
 }
 
@@ -1703,15 +1807,15 @@ __global__ __launch_bounds__(WAVE *NW) __attribute__((amdgpu_waves_per_eu(5))) v
 	__shared__ u64 redu[2 * NW];
 	__shared__ i32 xch[4];
 	__shared__ i32 next_q;
-	Blk<NW> b{red, redu, xch};
-	Bufs lb = carve(arena, lc, false);
+	Blk<NW> b{(li32 *)red, (lu64 *)redu, (li32 *)xch};
+	LBufs lb = carve_lds((lu64 *)arena, lc);
 	lb.mw = mw;
-	Bufs hb = carve(scratch + (size_t)blockIdx.x * ws_elems_per_wg, hc, true);
+	HBufs hb = carve(scratch + (size_t)blockIdx.x * ws_elems_per_wg, hc);
 	__shared__ i32 vfill;
-	hb.vfill = &vfill;
+	hb.vfill = (li32 *)&vfill;
 	lb.VK = hb.VK, lb.VS = hb.VS, lb.vmask = hb.vmask, lb.vfill = hb.vfill;  // one visited set in both modes
 	const i32 q_lo = bin_off[bin], q_n = bin_off[bin + 1] - q_lo;
-	hb.sort_lds = arena;  // the LDS arena is free once a read has spilled
+	hb.sort_lds = (lu64 *)arena;  // the LDS arena is free once a read has spilled
 	hb.sort_cap = arena_sort_cap((size_t)arena_n);
 	hb.sort_n = arena_n;
 	hb.mw = mw;
@@ -1724,7 +1828,8 @@ __global__ __launch_bounds__(WAVE *NW) __attribute__((amdgpu_waves_per_eu(5))) v
 		if (qi >= q_n) break;
 		const i32 r = order ? order[q_lo + qi] : q_lo + qi;
 		const u64 t0 = wall_clock64();
-		const ReadOut o = align_read<NW, true, MIXED>(g, rd.seq + rd.off[r], rd.ql[r], (u32)rd.v0[r], rd.v1, lb, hb, b, vis, beat, r);
+		const ReadOut o = align_read<NW, true, MIXED>(g, rd.seq + rd.off[r], rd.ql[r], (u32)rd.v0[r], rd.v1, lb, hb, b,
+		                                              (lu64 *)vis, beat, r);
 		write_out<NW>(o, r, t0, out_ed, out_status, out_counts, b);
 		b.sync();
 	}
@@ -1760,8 +1865,8 @@ __global__ __launch_bounds__(WAVE *NW) __attribute__((amdgpu_waves_per_eu(4))) v
 	__shared__ i32 vfill;
 	constexpr i32 NT = Blk<NW>::NT;
 	static_assert(sizeof(SrvEntry) == 64, "SrvEntry is eight u64 words");
-	Blk<NW> b{red, redu, xch};
-	Bufs lb = carve(arena, a.lc, false);
+	Blk<NW> b{(li32 *)red, (lu64 *)redu, (li32 *)xch};
+	LBufs lb = carve_lds((lu64 *)arena, a.lc);
 	lb.mw = a.mw;
 	const i32 sort_cap = arena_sort_cap(lds_elems(a.lc));
 	GwfBeat *beat = a.beat ? a.beat + blockIdx.x : nullptr;
@@ -1850,15 +1955,16 @@ __global__ __launch_bounds__(WAVE *NW) __attribute__((amdgpu_waves_per_eu(4))) v
 		}
 		b.sync();
 		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-		Bufs hb = carve((u64 *)(a.pool + e.d_off + e.ws_off), e.hc, true);
-		hb.vfill = &vfill;
+		HBufs hb = carve((u64 *)(a.pool + e.d_off + e.ws_off), e.hc);
+		hb.vfill = (li32 *)&vfill;
 		lb.VK = hb.VK, lb.VS = hb.VS, lb.vmask = hb.vmask, lb.vfill = hb.vfill;  // one visited set in both modes
-		hb.sort_lds = arena;
+		hb.sort_lds = (lu64 *)arena;
 		hb.sort_cap = sort_cap;
 		hb.sort_n = (i32)lds_elems(a.lc);
 		hb.mw = a.mw;
 		const u64 t0 = wall_clock64();
-		const ReadOut o = align_read<NW, true, MIXED>(a.g, dq, e.ql, (u32)e.v0, e.v1, lb, hb, b, vis, beat, (i32)seq);
+		const ReadOut o = align_read<NW, true, MIXED>(a.g, dq, e.ql, (u32)e.v0, e.v1, lb, hb, b, (lu64 *)vis, beat,
+		                                              (i32)seq);
 		int64_t c = o.chars;
 #pragma unroll
 		for (int s = 32; s > 0; s >>= 1) c += __shfl_xor(c, s);
