@@ -581,10 +581,10 @@ __device__ __forceinline__ void merge_intervals2(PA alo, PA ahi, i32 na, PC clo,
 // The same merge, partitioned into contiguous ranges of the output: one
 // merge-path search per thread, then a sequential merge (for lists in HBM,
 // where a binary search per element is a chain of dependent loads).
-template <int NW, class PA, class PC, class PO>
+template <class B, class PA, class PC, class PO>
 __device__ __forceinline__ void merge_intervals_ranges(PA alo, PA ahi, i32 na, PC clo, PC chi, i32 nc, PO olo, PO ohi,
-                                                       const Blk<NW> &b) {
-	constexpr i32 NT = Blk<NW>::NT;
+                                                       const B &b) {
+	constexpr i32 NT = B::NT;
 	const i32 n = na + nc;
 	const i32 E = (n + NT - 1) / NT;
 	const i32 o0 = (i32)b.tid() * E;
@@ -1472,6 +1472,28 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 			hist_res = 2 * cap_h;
 			q_res = n_s <= w.sort_n - 2 * cap_h ? n_s : 0;
 			Hl_l = w.sort_lds + w.sort_n - 2 * cap_h, Hh_l = Hl_l + cap_h;
+			p5_done = true;
+		} else if (v1 < 0 && 4 * n_t + 2 * n_h <= cap.m && cap_h <= cap.h) {
+			// the same split with the history update in HBM (P, free during the
+			// closure, holds its intermediate lists)
+			const WaveB wb;
+			if (b.wid() == 0) {
+				i32 nx = 0, nt = n_t;
+				const int rc = closure<MIXED>(g, q, ql, w, wb, vis, v1, n_f, nx, nt, chars, lvl);
+				if (lane_id() == 0) b.xch[0] = rc, b.xch[1] = nx;
+			} else if (b.wid() == 1) {
+				E *TIlo = w.P, *TIhi = w.P + n_t;
+				const i32 n_ti = ti_runs(w.T, n_t, TIlo, TIhi, wb);
+				wb.sync();
+				E *Clo = w.P + 2 * n_t, *Chi = Clo + (n_h + n_ti);
+				merge_intervals_ranges(Hcur_lo, Hcur_hi, n_h, TIlo, TIhi, n_ti, Clo, Chi, wb);
+				const i32 h2 = coalesce(Clo, Chi, n_h + n_ti, Hlo, Hhi, wb);
+				if (lane_id() == 0) b.xch[2] = h2;
+			}
+			b.sync();
+			const int rc = b.xch[0];
+			n_x = b.xch[1], nh2 = b.xch[2];
+			if (rc != STEP_NEXT) return rc;
 			p5_done = true;
 		}
 	}
