@@ -68,6 +68,13 @@ __device__ __forceinline__ i32 key_d(u64 x) { return (i32)((x >> KL.bk) & D_MASK
 __device__ __forceinline__ i32 key_k(u64 x) { return KINV_MAX - (i32)(x & K_MASK); }
 __device__ __forceinline__ u64 key_diag(u64 x) { return x >> KL.bk; }
 __device__ __forceinline__ u64 diag_id(u32 v, i32 d) { return (u64)v << KL.bd | (u64)(u32)(d + D_BIAS); }
+// The same state with offset k (d and v unchanged).
+__device__ __forceinline__ u64 with_k(u64 x, i32 k) { return (x & ~(u64)K_MASK) | (u64)(u32)(KINV_MAX - k); }
+// One diagonal in key units.  A state's d + D_BIAS is at least 1 (d >= -(vertex
+// length) > -D_BIAS) and a successor on d + 1 stays inside the read (d + 1 <
+// read length < D_BIAS), so key +- D_ONE never carries into or borrows from the
+// vertex field.
+constexpr u64 D_ONE = (u64)1 << KL.bk;
 
 // ------------------------------------------------------------ wave helpers
 __device__ __forceinline__ u32 lane_id() { return __lane_id(); }
@@ -935,21 +942,24 @@ struct Succ {
 	i32 k0, k1, k2;
 };
 __device__ __forceinline__ Succ successors(bool act, u64 cur, u64 prv, u64 nxt, i32 vl, i32 ql) {
-	const u32 v = key_v(cur);
+	// neighbours compared by diagonal id (vertex and diagonal in one number;
+	// consecutive diagonals of a vertex have consecutive ids)
+	const u64 cdg = key_diag(cur), pdg = key_diag(prv), ndg = key_diag(nxt);
 	const i32 d = key_d(cur), k = key_k(cur);
-	const bool hp = act && key_v(prv) == v;  // EMPTY_KEY never matches a vertex
-	const bool hn = act && key_v(nxt) == v;
-	const i32 dp = key_d(prv), kp = key_k(prv), dn = key_d(nxt), kn = key_k(nxt);
+	const bool hp = act && (pdg >> KL.bd) == (cdg >> KL.bd);  // EMPTY_KEY never matches a vertex
+	const bool hn = act && (ndg >> KL.bd) == (cdg >> KL.bd);
+	const i32 kp = key_k(prv), kn = key_k(nxt);
 	Succ o;
-	const bool e0 = act && !(hp && dp >= d - 2);
+	const bool p1 = hp && pdg + 1 == cdg, n1 = hn && ndg == cdg + 1;
+	const bool e0 = act && !(hp && pdg + 2 >= cdg);
 	o.k0 = k + 1;
 	o.k1 = k + 1;
-	if (hp && dp == d - 1 && kp > o.k1) o.k1 = kp;
-	if (hn && dn == d + 1 && kn + 1 > o.k1) o.k1 = kn + 1;
+	if (p1 && kp > o.k1) o.k1 = kp;
+	if (n1 && kn + 1 > o.k1) o.k1 = kn + 1;
 	const bool e1 = act;
-	const bool e2 = act && !(hn && dn == d + 1);
+	const bool e2 = act && !n1;
 	o.k2 = k;
-	if (hn && dn == d + 2 && kn + 1 > o.k2) o.k2 = kn + 1;
+	if (hn && ndg == cdg + 2 && kn + 1 > o.k2) o.k2 = kn + 1;
 	o.s0 = e0 && (d - 1) + o.k0 < ql && o.k0 < vl, o.t0 = e0 && o.k0 == vl;
 	o.s1 = e1 && d + o.k1 < ql && o.k1 < vl, o.t1 = e1 && o.k1 == vl;
 	o.s2 = e2 && (d + 1) + o.k2 < ql && o.k2 < vl, o.t2 = e2 && o.k2 == vl;
@@ -1357,7 +1367,7 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 				const u32 v = key_v(key[s]);
 				const i32 d = key_d(key[s]), k = kk[s], vl = (i32)vi[s].y;
 				if (act) {
-					nk[s] = mk(v, d, k);
+					nk[s] = with_k(key[s], k);
 					nvl[s] = vl;
 					// LDS arrays: keep the extension, so a spilled step restarts from it
 					if constexpr (!HBM) w.P[j0 + s] = nk[s];
@@ -1407,11 +1417,10 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 			else {
 #pragma unroll
 				for (i32 s = 0; s < S; ++s) {
-					const u32 v = key_v(c[s + 1]);
-					const i32 d = key_d(c[s + 1]);
-					if (sc[s].s0) w.Q[p++] = mk(v, d - 1, sc[s].k0);
-					if (sc[s].s1) w.Q[p++] = mk(v, d, sc[s].k1);
-					if (sc[s].s2) w.Q[p++] = mk(v, d + 1, sc[s].k2);
+					const u64 cur = c[s + 1];
+					if (sc[s].s0) w.Q[p++] = with_k(cur - D_ONE, sc[s].k0);
+					if (sc[s].s1) w.Q[p++] = with_k(cur, sc[s].k1);
+					if (sc[s].s2) w.Q[p++] = with_k(cur + D_ONE, sc[s].k2);
 				}
 			}
 			n_s += tot;
@@ -1420,11 +1429,10 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 			else {
 #pragma unroll
 				for (i32 s = 0; s < S; ++s) {
-					const u32 v = key_v(c[s + 1]);
-					const i32 d = key_d(c[s + 1]);
-					if (sc[s].t0) w.T[p++] = diag_id(v, d - 1);
-					if (sc[s].t1) w.T[p++] = diag_id(v, d);
-					if (sc[s].t2) w.T[p++] = diag_id(v, d + 1);
+					const u64 cdg = key_diag(c[s + 1]);
+					if (sc[s].t0) w.T[p++] = cdg - 1;
+					if (sc[s].t1) w.T[p++] = cdg;
+					if (sc[s].t2) w.T[p++] = cdg + 1;
 				}
 			}
 			n_t += tot_t;
