@@ -456,18 +456,22 @@ __device__ __forceinline__ u64 wave_sort64(u64 x) {
 
 // One round of pairwise merges of the sorted runs of width wdt in src[0..n)
 // into dst (left run first on ties).  Thread t writes outputs [t E, t E + E)
-// (E a power of two dividing 2 wdt, so a range never crosses a pair): one
-// merge-path search, then a sequential merge from the two run heads.
+// with E odd (above 2): a power-of-two stride between the lanes' reads and
+// writes would put a wave's accesses on a few LDS banks.  A range that crosses
+// the boundary of a pair is merged piece by piece: per piece one merge-path
+// search, then a sequential merge from the two run heads.
 template <int NW, class PS, class PD>
 __device__ __forceinline__ void merge_round(PS src, PD dst, i32 n, i32 wdt, const Blk<NW> &b) {
 	constexpr i32 NT = Blk<NW>::NT;
-	i32 E = 1;
-	while (E * NT < n && E < 2 * wdt) E <<= 1;
-	for (i32 o0 = (i32)b.tid() * E; o0 < n; o0 += NT * E) {
-		const i32 l = o0 / (2 * wdt) * (2 * wdt);
+	i32 E = (n + NT - 1) / NT;
+	if (E > 2) E |= 1;
+	const i32 o0 = (i32)b.tid() * E;
+	const i32 o1 = o0 + E < n ? o0 + E : n;
+	for (i32 ob = o0; ob < o1;) {
+		const i32 l = ob / (2 * wdt) * (2 * wdt);
 		const i32 mid = l + wdt < n ? l + wdt : n, r = l + 2 * wdt < n ? l + 2 * wdt : n;
 		const PS A = src + l, C = src + mid;
-		const i32 na = mid - l, nc = r - mid, dg = o0 - l;
+		const i32 na = mid - l, nc = r - mid, dg = ob - l;
 		i32 lo = dg - nc > 0 ? dg - nc : 0, hi = dg < na ? dg : na;
 		while (lo < hi) {
 			const i32 m = (lo + hi) >> 1;
@@ -476,8 +480,8 @@ __device__ __forceinline__ void merge_round(PS src, PD dst, i32 n, i32 wdt, cons
 		}
 		i32 i = lo, j = dg - lo;
 		u64 ah = i < na ? A[i] : EMPTY_KEY, ch = j < nc ? C[j] : EMPTY_KEY;
-		const i32 oe = o0 + E < r ? o0 + E : r;
-		for (i32 o = o0; o < oe; ++o) {
+		const i32 oe = o1 < r ? o1 : r;
+		for (i32 o = ob; o < oe; ++o) {
 			const bool ta = ah <= ch;  // an exhausted run's head is EMPTY_KEY, above every key
 			dst[o] = ta ? ah : ch;
 			if (ta) {
@@ -488,6 +492,7 @@ __device__ __forceinline__ void merge_round(PS src, PD dst, i32 n, i32 wdt, cons
 				ch = j < nc ? C[j] : EMPTY_KEY;
 			}
 		}
+		ob = oe;
 	}
 }
 
