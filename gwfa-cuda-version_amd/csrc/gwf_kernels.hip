@@ -878,10 +878,39 @@ struct BufsT {
 	u64 *VK, *VS;
 	u32 vmask;        // slots - 1
 	li32 *vfill;      // LDS: slots taken
+	// HBM mode: the first frc entries of the frontier lists F0 and F1 live in
+	// the bottom of the LDS scratch (FL0, FL1; 2 frc elements, kept free until
+	// the closure ends), so a closure level's small lists make no HBM round trip
+	lu64 *FL0, *FL1;
+	i32 frc;
 	Caps c;
 };
 typedef BufsT<lu64> LBufs;
 typedef BufsT<u64> HBufs;
+template <class E> struct IsHbm { static constexpr bool v = false; };
+template <> struct IsHbm<u64> { static constexpr bool v = true; };
+#define FRC 64  // frontier entries per list kept in the LDS scratch in HBM mode
+
+// Entry j of a frontier list (A: F0 or F1 in the workspace, L: its LDS part).
+template <class E>
+__device__ __forceinline__ u64 front_get(const BufsT<E> &w, const E *A, const lu64 *L, i32 j) {
+	if constexpr (IsHbm<E>::v) return j < w.frc ? L[j] : A[j];
+	else return A[j];
+}
+template <class E>
+__device__ __forceinline__ void front_put(const BufsT<E> &w, E *A, lu64 *L, i32 j, u64 x) {
+	if constexpr (IsHbm<E>::v) {
+		if (j < w.frc) L[j] = x;
+		else A[j] = x;
+	} else {
+		A[j] = x;
+	}
+}
+// Give the frontier lists the bottom 2 FRC elements of an HBM workspace's LDS
+// scratch when it holds at least 8 times that.
+__device__ __forceinline__ void front_in_scratch(BufsT<u64> &w) {
+	if (w.sort_n >= 16 * FRC) w.FL0 = w.sort_lds, w.FL1 = w.sort_lds + FRC, w.frc = FRC;
+}
 
 __host__ __device__ inline size_t ws_elems(const Caps &c) {
 	return 2 * (size_t)c.m + c.x + 2 * (size_t)c.f + c.t + 4 * (size_t)c.h + (size_t)(c.x > c.t ? c.x : c.t) + 8 +
@@ -910,6 +939,8 @@ __device__ inline void carve_arrays(BufsT<E> &w, E *p, const Caps &c) {
 	w.VK = w.VS = nullptr;
 	w.vmask = 0;
 	w.vfill = nullptr;
+	w.FL0 = w.FL1 = nullptr;
+	w.frc = 0;
 	w.sort_lds = nullptr;
 	w.sort_n = 0;
 	w.mw = 8;
@@ -976,15 +1007,16 @@ enum { STEP_NEXT = 0, STEP_FOUND = 1, STEP_EMPTY = 2, STEP_SPILL = 3 };
 // P5 for a spilled read, in the LDS scratch: exit runs of T[0..n_tt),
 // current history (n_h intervals), merged list; the new history goes to the
 // top of the scratch (2 (n_h + n_tt) elements reserved there), where it
-// stays for the filter, and to Hlo/Hhi in HBM for the next step.  Needs
-// 2 n_tt + 2 n_h + 4 (n_h + n_tt) <= sort_n.  Returns its interval count.
+// stays for the filter, and to Hlo/Hhi in HBM for the next step; the bottom
+// `base` elements are left alone.  Needs base + 2 n_tt + 2 n_h + 4 (n_h +
+// n_tt) <= sort_n.  Returns its interval count.
 template <class B>
 __device__ __forceinline__ i32 hist_lds(const HBufs &w, const B &b, const u64 *Hc_lo, const u64 *Hc_hi, i32 n_h, i32 n_tt,
-                                        u64 *Hlo, u64 *Hhi) {
+                                        u64 *Hlo, u64 *Hhi, i32 base) {
 	const i32 cap_h = n_h + n_tt;
-	lu64 *l = w.sort_lds;
+	lu64 *l = w.sort_lds + base;
 	lu64 *hc = l + 2 * n_tt, *Clo = hc + 2 * n_h, *Chi = Clo + cap_h;
-	lu64 *Olo = l + w.sort_n - 2 * cap_h, *Ohi = Olo + cap_h;
+	lu64 *Olo = w.sort_lds + w.sort_n - 2 * cap_h, *Ohi = Olo + cap_h;
 	stage(hc, Hc_lo, n_h, b);
 	stage(hc + n_h, Hc_hi, n_h, b);
 	const i32 n_ti = ti_runs(w.T, n_tt, l, l + n_tt, b);
@@ -1018,7 +1050,7 @@ __device__ __forceinline__ bool closure_mixed(const DGraph &g, const uint8_t *q,
 	for (i32 base = 0; base < n_f; base += NT) {
 		const i32 j = base + tid;
 		const bool act = j < n_f;
-		const u64 key = act ? w.F0[j] : 0;
+		const u64 key = act ? front_get(w, w.F0, w.FL0, j) : 0;
 		const i32 i = key_d(key) + key_k(key);
 		const uint4 vi = act ? g.vinfo[key_v(key)] : make_uint4(0, 0, 0, 0);
 		const i32 deg = act && i + 1 != ql ? (i32)vi.w : 0;  // a query end claims no key
@@ -1039,7 +1071,7 @@ __device__ __forceinline__ bool closure_mixed(const DGraph &g, const uint8_t *q,
 	for (i32 base = 0; base < n_f; base += NT) {
 		const i32 j = base + tid;
 		const bool act = j < n_f;
-		const u64 key = act ? w.F0[j] : 0;
+		const u64 key = act ? front_get(w, w.F0, w.FL0, j) : 0;
 		const u32 v = key_v(key);
 		const i32 d = key_d(key), k = key_k(key), i = d + k;
 		const uint4 vi = act ? g.vinfo[v] : make_uint4(0, 0, 0, 0);
@@ -1091,7 +1123,7 @@ __device__ __forceinline__ bool closure_mixed(const DGraph &g, const uint8_t *q,
 			const bool m = (uint8_t)(ar.y >> 24) == c;
 			if (!owns(wv, e)) continue;
 			if (m) {
-				if (sa < cap.f) w.F1[sa] = mk(wv, i + 1 - ol, ol);
+				if (sa < cap.f) front_put(w, w.F1, w.FL1, sa, mk(wv, i + 1 - ol, ol));
 				++sa;
 			} else {
 				if (sx + 1 < cap.x) {
@@ -1134,7 +1166,7 @@ __device__ __forceinline__ int closure(const DGraph &g, const uint8_t *q, i32 ql
 		for (i32 base = 0; !MIXED && base < n_f; base += NT) {
 			const i32 j = base + tid;
 			const bool act = j < n_f;
-			const u64 key = act ? w.F0[j] : 0;
+			const u64 key = act ? front_get(w, w.F0, w.FL0, j) : 0;
 			const u32 v = key_v(key);
 			const i32 d = key_d(key), k = key_k(key);
 			const i32 i = d + k;
@@ -1184,7 +1216,7 @@ __device__ __forceinline__ int closure(const DGraph &g, const uint8_t *q, i32 ql
 					const u32 wv = ar[u].x;
 					const i32 ol = (i32)(ar[u].y & 0xFFFFFFu);
 					if ((pa_m >> u) & 1) {
-						if (sa < cap.f) w.F1[sa] = mk(wv, i + 1 - ol, ol);
+						if (sa < cap.f) front_put(w, w.F1, w.FL1, sa, mk(wv, i + 1 - ol, ol));
 						++sa;
 					}
 					if ((px_m >> u) & 1) {
@@ -1217,7 +1249,7 @@ __device__ __forceinline__ int closure(const DGraph &g, const uint8_t *q, i32 ql
 		for (i32 base = 0; base < n_na; base += NT) {
 			const i32 j = base + tid;
 			const bool act = j < n_na;
-			const u64 key = act ? w.F1[j] : 0;
+			const u64 key = act ? front_get(w, w.F1, w.FL1, j) : 0;
 			const u32 v = key_v(key);
 			const i32 d = key_d(key);
 			i32 k = key_k(key);
@@ -1236,7 +1268,7 @@ __device__ __forceinline__ int closure(const DGraph &g, const uint8_t *q, i32 ql
 			sx += n_x;
 			n_f2 += tf;
 			n_x += tx;
-			if (vend && sf < cap.f) w.F0[sf] = mk(v, d, k);
+			if (vend && sf < cap.f) front_put(w, w.F0, w.FL0, sf, with_k(key, k));
 			if (mid && sx + 2 < cap.x) {
 				w.X[sx] = mk(v, d - 1, k + 1);
 				w.X[sx + 1] = mk(v, d, k + 1);
@@ -1393,7 +1425,7 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 #pragma unroll
 			for (i32 s = 0; s < S; ++s)
 				if (vend[s]) {
-					if (pos < cap.f) w.F0[pos] = nk[s];
+					if (pos < cap.f) front_put(w, w.F0, w.FL0, pos, nk[s]);
 					++pos;
 				}
 			n_f += tot;
@@ -1466,16 +1498,18 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 	bool p5_done = false;
 	if constexpr (HBM && NW == 2) {
 		const i32 cap_h = n_h + n_t;
-		if (v1 < 0 && 2 * n_t + 2 * n_h + 4 * cap_h <= w.sort_n && cap_h <= cap.h) {
+		// (the bottom 2 frc elements of the scratch hold the closure's frontier lists)
+		const i32 fr = 2 * w.frc;
+		if (v1 < 0 && fr + 2 * n_t + 2 * n_h + 4 * cap_h <= w.sort_n && cap_h <= cap.h) {
 			const WaveB wb;
 			if (b.wid() == 0) {
 				i32 nx = 0, nt = n_t;
 				const int rc = closure<MIXED>(g, q, ql, w, wb, vis, v1, n_f, nx, nt, chars, lvl);
 				if (lane_id() == 0) b.xch[0] = rc, b.xch[1] = nx;
 			} else if (b.wid() == 1) {
-				const i32 h2 = hist_lds(w, wb, Hcur_lo, Hcur_hi, n_h, n_t, Hlo, Hhi);
+				const i32 h2 = hist_lds(w, wb, Hcur_lo, Hcur_hi, n_h, n_t, Hlo, Hhi, fr);
 				// and Q, for the merge, right below the new history
-				if (n_s <= w.sort_n - 2 * cap_h) stage(w.sort_lds + w.sort_n - 2 * cap_h - n_s, w.Q, n_s, wb);
+				if (fr + n_s <= w.sort_n - 2 * cap_h) stage(w.sort_lds + w.sort_n - 2 * cap_h - n_s, w.Q, n_s, wb);
 				if (lane_id() == 0) b.xch[2] = h2;
 			}
 			b.sync();
@@ -1483,7 +1517,7 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 			n_x = b.xch[1], nh2 = b.xch[2];
 			if (rc != STEP_NEXT) return rc;
 			hist_res = 2 * cap_h;
-			q_res = n_s <= w.sort_n - 2 * cap_h ? n_s : 0;
+			q_res = fr + n_s <= w.sort_n - 2 * cap_h ? n_s : 0;
 			Hl_l = w.sort_lds + w.sort_n - 2 * cap_h, Hh_l = Hl_l + cap_h;
 			p5_done = true;
 		} else if (v1 < 0 && 4 * n_t + 2 * n_h <= cap.m && cap_h <= cap.h) {
@@ -1553,7 +1587,7 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 	const i32 cap_h = n_h + n_tt;  // bound on the merged interval count
 	if constexpr (HBM) {
 		if (!p5_done && 2 * n_tt + 2 * n_h + 4 * cap_h <= w.sort_n) {
-			nh2 = hist_lds(w, b, Hcur_lo, Hcur_hi, n_h, n_tt, Hlo, Hhi);
+			nh2 = hist_lds(w, b, Hcur_lo, Hcur_hi, n_h, n_tt, Hlo, Hhi, 0);
 			hist_res = 2 * cap_h;
 			Hl_l = w.sort_lds + w.sort_n - 2 * cap_h, Hh_l = Hl_l + cap_h;
 			p5_done = true;
@@ -1854,6 +1888,7 @@ __global__ __launch_bounds__(WAVE *NW) __attribute__((amdgpu_waves_per_eu(5))) v
 	hb.sort_cap = arena_sort_cap((size_t)arena_n);
 	hb.sort_n = arena_n;
 	hb.mw = mw;
+	front_in_scratch(hb);
 	GwfBeat *beat = beats ? beats + blockIdx.x : nullptr;
 	for (;;) {
 		if (b.tid() == 0) next_q = atomicAdd(&heads[bin], 1);
@@ -1997,6 +2032,7 @@ __global__ __launch_bounds__(WAVE *NW) __attribute__((amdgpu_waves_per_eu(4))) v
 		hb.sort_cap = sort_cap;
 		hb.sort_n = (i32)lds_elems(a.lc);
 		hb.mw = a.mw;
+		front_in_scratch(hb);
 		const u64 t0 = wall_clock64();
 		const ReadOut o = align_read<NW, true, MIXED>(a.g, dq, e.ql, (u32)e.v0, e.v1, lb, hb, b, (lu64 *)vis, beat,
 		                                              (i32)seq);
