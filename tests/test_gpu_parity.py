@@ -125,6 +125,43 @@ def test_random_graphs_vs_oracle(gpu, orc, seed, dag, ovl, kernel):
 
 
 @pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("seed,dag,ovl", [(11, True, False), (12, False, True)])
+def test_high_degree_vertices_vs_oracle(gpu, orc, seed, dag, ovl, kernel):
+    """Vertices with up to 11 out-arcs (the closure takes a state's arcs in
+    groups of four, so several groups per state), parallel alleles of one or
+    a few bases, against the oracle."""
+    _say(f"case: high-degree graphs seed={seed} dag={dag} ovl={ovl} kernel {kernel}")
+    rng = random.Random(seed)
+    for it in range(12):
+        nv = rng.randint(6, 40)
+        seqs = ["".join(rng.choice("ACGT") for _ in range(rng.randint(1, 4 if rng.random() < 0.6 else 25)))
+                for _ in range(nv)]
+        fixed_o = [((w * 5 + 1) % len(seqs[w])) if ovl else 0 for w in range(nv)]
+        arcs = set()
+        for v in range(nv):
+            for _ in range(rng.randint(0, 11)):
+                if dag:
+                    if v + 1 >= nv:
+                        break
+                    w = rng.randint(v + 1, nv - 1)
+                else:
+                    w = rng.randrange(nv)
+                arcs.add((v, w))
+        g = gpu.Graph.from_lists(seqs, [(v, w, fixed_o[w]) for v, w in sorted(arcs)])
+        arcs_of = {}
+        for v, w, o in g.arcs():
+            arcs_of.setdefault(v, []).append((w, o))
+        assert max(len(a) for a in arcs_of.values()) > 4
+        reads = [_sample(rng, seqs, arcs_of, rng.randint(1, 250), rng.choice([0.0, 0.02, 0.1, 0.3]))
+                 for _ in range(24)]
+        r = gpu.Reads.from_list(reads)
+        g.index()
+        want, _, _ = orc.batch(g, r)
+        got, _ = _gpu_eds(gpu, g, r, kernel)
+        assert np.array_equal(got, want), f"iteration {it}: gpu {got.tolist()} oracle {want.tolist()}"
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
 def test_edge_cases(gpu, orc, kernel):
     _say(f"case: edge cases, kernel {kernel}")
     # short start vertex (< 32 bp) read as an exact prefix: the reference's
