@@ -829,15 +829,15 @@ __device__ __forceinline__ u32 vis_hash(u64 x) {
 }
 // true if key was absent (and is now present); a full probe window reports
 // "absent", which only costs duplicate work.
-__device__ __forceinline__ bool vis_insert(lu64 *vis, u64 key) {
-	u32 h = vis_hash(key) & (VIS_SLOTS - 1);
+__device__ __forceinline__ bool vis_insert(lu64 *vis, u32 vmask, u64 key) {
+	u32 h = vis_hash(key) & vmask;
 	for (int p = 0; p < 16; ++p) {
 		u64 old = EMPTY_KEY;
 		if (__hip_atomic_compare_exchange_strong(&vis[h], &old, key, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
 		                                         __HIP_MEMORY_SCOPE_WORKGROUP))
 			return true;
 		if (old == key) return false;
-		h = (h + 1) & (VIS_SLOTS - 1);
+		h = (h + 1) & vmask;
 	}
 	return true;
 }
@@ -1150,12 +1150,13 @@ __device__ __forceinline__ bool closure_mixed(const DGraph &g, const uint8_t *q,
 // (Blk) or one wave (WaveB).
 template <bool MIXED, class B, class E>
 __device__ __forceinline__ int closure(const DGraph &g, const uint8_t *q, i32 ql, const BufsT<E> &w, const B &b, lu64 *vis,
+                                       u32 vmask,
                                        i32 v1, i32 n_f, i32 &n_x, i32 &n_t, int64_t &chars, u64 &lvl) {
 	constexpr i32 NT = B::NT;
 	const i32 tid = (i32)b.tid();
 	const Caps &cap = w.c;
 	bool found = false;
-	for (i32 t = tid; t < VIS_SLOTS; t += NT) vis[t] = EMPTY_KEY;
+	for (i32 t = tid; t <= (i32)vmask; t += NT) vis[t] = EMPTY_KEY;
 	b.sync();
 	bool closure_origin = false;
 	while (n_f > 0) {
@@ -1199,7 +1200,7 @@ __device__ __forceinline__ int closure(const DGraph &g, const uint8_t *q, i32 ql
 					const bool has = e0 + u < deg;
 					const bool m = has && !qe && (uint8_t)(ar[u].y >> 24) == c;
 					n_ext += m;
-					const bool absent = has && !qe && vis_insert(vis, (u64)ar[u].x << 32 | (u32)(i + 1));
+					const bool absent = has && !qe && vis_insert(vis, vmask, (u64)ar[u].x << 32 | (u32)(i + 1));
 					pa_m |= (absent && m ? 1u : 0u) << u;
 					px_m |= (absent && !m ? 1u : 0u) << u;
 					pq_m |= (has && qe ? 1u : 0u) << u;
@@ -1504,7 +1505,7 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 			const WaveB wb;
 			if (b.wid() == 0) {
 				i32 nx = 0, nt = n_t;
-				const int rc = closure<MIXED>(g, q, ql, w, wb, vis, v1, n_f, nx, nt, chars, lvl);
+				const int rc = closure<MIXED>(g, q, ql, w, wb, vis, VIS_SLOTS - 1, v1, n_f, nx, nt, chars, lvl);
 				if (lane_id() == 0) b.xch[0] = rc, b.xch[1] = nx;
 			} else if (b.wid() == 1) {
 				const i32 h2 = hist_lds(w, wb, Hcur_lo, Hcur_hi, n_h, n_t, Hlo, Hhi, fr);
@@ -1526,7 +1527,7 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 			const WaveB wb;
 			if (b.wid() == 0) {
 				i32 nx = 0, nt = n_t;
-				const int rc = closure<MIXED>(g, q, ql, w, wb, vis, v1, n_f, nx, nt, chars, lvl);
+				const int rc = closure<MIXED>(g, q, ql, w, wb, vis, VIS_SLOTS - 1, v1, n_f, nx, nt, chars, lvl);
 				if (lane_id() == 0) b.xch[0] = rc, b.xch[1] = nx;
 			} else if (b.wid() == 1) {
 				E *TIlo = w.P, *TIhi = w.P + n_t;
@@ -1545,7 +1546,18 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 		}
 	}
 	if (!p5_done) {
-		const int rc = closure<MIXED>(g, q, ql, w, b, vis, v1, n_f, n_x, n_t, chars, lvl);
+		// the per-step visited set: the static VIS_SLOTS, or in HBM mode, where
+		// the LDS scratch is free during a block-wide closure, up to 16x that
+		// in the scratch (above the frontier lists)
+		lu64 *vset = vis;
+		u32 vmask = VIS_SLOTS - 1;
+		if constexpr (HBM) {
+			const i32 room = w.sort_n - 2 * w.frc;
+			u32 slots = VIS_SLOTS;
+			while (slots < 16 * VIS_SLOTS && (i32)(2 * slots) <= room) slots *= 2;
+			if (slots > VIS_SLOTS) vset = w.sort_lds + 2 * w.frc, vmask = slots - 1;
+		}
+		const int rc = closure<MIXED>(g, q, ql, w, b, vset, vmask, v1, n_f, n_x, n_t, chars, lvl);
 		if (rc != STEP_NEXT) return rc;
 	}
 
