@@ -1287,6 +1287,19 @@ __device__ __forceinline__ int closure(const DGraph &g, const uint8_t *q, i32 ql
 	return STEP_NEXT;
 }
 
+// The per-step visited set of a closure: the static one (vis, VIS_SLOTS
+// slots), or, when the LDS scratch from element `base` on is free during the
+// closure, the largest power of two of slots up to 16 VIS_SLOTS that fits
+// there (a set too small for a large frontier lets duplicate entries
+// through: every claim beyond its probe window counts as new).  Returns the
+// slot mask.
+__device__ __forceinline__ u32 step_vset(const HBufs &w, i32 base, lu64 *vis, lu64 *&vset) {
+	u32 slots = VIS_SLOTS;
+	while (slots < 16 * VIS_SLOTS && (i32)(2 * slots) <= w.sort_n - base) slots *= 2;
+	vset = slots > VIS_SLOTS ? w.sort_lds + base : vis;
+	return slots - 1;
+}
+
 // One score step.  Reads P[0..n_a) (the wavefront, sorted) and the history
 // H[hs] (n_h intervals); on STEP_NEXT leaves the next wavefront in P and the
 // updated history in H[hs ^ 1].  On STEP_SPILL nothing the next attempt needs
@@ -1523,11 +1536,14 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 			p5_done = true;
 		} else if (v1 < 0 && 4 * n_t + 2 * n_h <= cap.m && cap_h <= cap.h) {
 			// the same split with the history update in HBM (P, free during the
-			// closure, holds its intermediate lists)
+			// closure, holds its intermediate lists; the LDS scratch above the
+			// frontier lists holds the closure's visited set)
 			const WaveB wb;
 			if (b.wid() == 0) {
 				i32 nx = 0, nt = n_t;
-				const int rc = closure<MIXED>(g, q, ql, w, wb, vis, VIS_SLOTS - 1, v1, n_f, nx, nt, chars, lvl);
+				lu64 *vset;
+				const u32 vmask = step_vset(w, fr, vis, vset);
+				const int rc = closure<MIXED>(g, q, ql, w, wb, vset, vmask, v1, n_f, nx, nt, chars, lvl);
 				if (lane_id() == 0) b.xch[0] = rc, b.xch[1] = nx;
 			} else if (b.wid() == 1) {
 				E *TIlo = w.P, *TIhi = w.P + n_t;
@@ -1546,17 +1562,10 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 		}
 	}
 	if (!p5_done) {
-		// the per-step visited set: the static VIS_SLOTS, or in HBM mode, where
-		// the LDS scratch is free during a block-wide closure, up to 16x that
-		// in the scratch (above the frontier lists)
+		// (the LDS scratch is free during a block-wide closure)
 		lu64 *vset = vis;
 		u32 vmask = VIS_SLOTS - 1;
-		if constexpr (HBM) {
-			const i32 room = w.sort_n - 2 * w.frc;
-			u32 slots = VIS_SLOTS;
-			while (slots < 16 * VIS_SLOTS && (i32)(2 * slots) <= room) slots *= 2;
-			if (slots > VIS_SLOTS) vset = w.sort_lds + 2 * w.frc, vmask = slots - 1;
-		}
+		if constexpr (HBM) vmask = step_vset(w, 2 * w.frc, vis, vset);
 		const int rc = closure<MIXED>(g, q, ql, w, b, vset, vmask, v1, n_f, n_x, n_t, chars, lvl);
 		if (rc != STEP_NEXT) return rc;
 	}
