@@ -1514,16 +1514,27 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 		const i32 cap_h = n_h + n_t;
 		// (the bottom 2 frc elements of the scratch hold the closure's frontier lists)
 		const i32 fr = 2 * w.frc;
-		if (v1 < 0 && fr + 2 * n_t + 2 * n_h + 4 * cap_h <= w.sort_n && cap_h <= cap.h) {
+		const i32 hneed = 2 * n_t + 2 * n_h + 4 * cap_h;
+		if (v1 < 0 && fr + hneed <= w.sort_n && cap_h <= cap.h) {
+			// a large frontier takes a visited set above the frontier lists
+			// when the room left beside the history update holds 512 slots
+			i32 vs = 0;
+			lu64 *vset = vis;
+			u32 vmask = VIS_SLOTS - 1;
+			if (n_f > 64 && w.sort_n - fr - hneed >= 2 * VIS_SLOTS) {
+				vs = 2 * VIS_SLOTS;
+				while (vs < 16 * VIS_SLOTS && 2 * vs <= w.sort_n - fr - hneed) vs *= 2;
+				vset = w.sort_lds + fr, vmask = (u32)vs - 1;
+			}
 			const WaveB wb;
 			if (b.wid() == 0) {
 				i32 nx = 0, nt = n_t;
-				const int rc = closure<MIXED>(g, q, ql, w, wb, vis, VIS_SLOTS - 1, v1, n_f, nx, nt, chars, lvl);
+				const int rc = closure<MIXED>(g, q, ql, w, wb, vset, vmask, v1, n_f, nx, nt, chars, lvl);
 				if (lane_id() == 0) b.xch[0] = rc, b.xch[1] = nx;
 			} else if (b.wid() == 1) {
-				const i32 h2 = hist_lds(w, wb, Hcur_lo, Hcur_hi, n_h, n_t, Hlo, Hhi, fr);
+				const i32 h2 = hist_lds(w, wb, Hcur_lo, Hcur_hi, n_h, n_t, Hlo, Hhi, fr + vs);
 				// and Q, for the merge, right below the new history
-				if (fr + n_s <= w.sort_n - 2 * cap_h) stage(w.sort_lds + w.sort_n - 2 * cap_h - n_s, w.Q, n_s, wb);
+				if (fr + vs + n_s <= w.sort_n - 2 * cap_h) stage(w.sort_lds + w.sort_n - 2 * cap_h - n_s, w.Q, n_s, wb);
 				if (lane_id() == 0) b.xch[2] = h2;
 			}
 			b.sync();
@@ -1531,7 +1542,7 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 			n_x = b.xch[1], nh2 = b.xch[2];
 			if (rc != STEP_NEXT) return rc;
 			hist_res = 2 * cap_h;
-			q_res = fr + n_s <= w.sort_n - 2 * cap_h ? n_s : 0;
+			q_res = fr + vs + n_s <= w.sort_n - 2 * cap_h ? n_s : 0;
 			Hl_l = w.sort_lds + w.sort_n - 2 * cap_h, Hh_l = Hl_l + cap_h;
 			p5_done = true;
 		} else if (v1 < 0 && 4 * n_t + 2 * n_h <= cap.m && cap_h <= cap.h) {
