@@ -1645,11 +1645,13 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 		i32 free_cap = 128;
 		while (free_cap * 2 <= free_n) free_cap <<= 1;
 		if (2 * n_x <= free_n) {  // stage X in LDS, sort it there and merge from there
-			stage(w.sort_lds, w.X, n_x, b);
+			// (staged above, sorted into the bottom: the staging half is free
+			// again for the history and Q)
+			stage(w.sort_lds + n_x, w.X, n_x, b);
 			b.sync();
-			sort_runs(w.sort_lds, n_x, w.sort_lds + n_x, w.mw, b);
-			Xl = w.sort_lds + n_x;
-			lds_used = 2 * n_x;
+			sort_runs(w.sort_lds + n_x, n_x, w.sort_lds, w.mw, b);
+			Xl = w.sort_lds;
+			lds_used = n_x;
 			x_lds = true;
 		} else if ((hist_res || q_res) && free_cap >= 256 && free_cap <= free_n) {
 			// X sorted in HBM through the free part of the scratch, so that the
