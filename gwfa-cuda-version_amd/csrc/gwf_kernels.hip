@@ -889,7 +889,7 @@ typedef BufsT<lu64> LBufs;
 typedef BufsT<u64> HBufs;
 template <class E> struct IsHbm { static constexpr bool v = false; };
 template <> struct IsHbm<u64> { static constexpr bool v = true; };
-#define FRC 64  // frontier entries per list kept in the LDS scratch in HBM mode
+#define FRC 64  // frontier entries per list kept in the LDS scratch in HBM mode (2 FRC in large scratches)
 
 // Entry j of a frontier list (A: F0 or F1 in the workspace, L: its LDS part).
 template <class E>
@@ -906,10 +906,11 @@ __device__ __forceinline__ void front_put(const BufsT<E> &w, E *A, lu64 *L, i32 
 		A[j] = x;
 	}
 }
-// Give the frontier lists the bottom 2 FRC elements of an HBM workspace's LDS
-// scratch when it holds at least 8 times that.
+// Give the frontier lists the bottom of an HBM workspace's LDS scratch: FRC
+// entries each when it holds 8 times that, 2 FRC when it holds 32 times FRC.
 __device__ __forceinline__ void front_in_scratch(BufsT<u64> &w) {
-	if (w.sort_n >= 16 * FRC) w.FL0 = w.sort_lds, w.FL1 = w.sort_lds + FRC, w.frc = FRC;
+	const i32 f = w.sort_n >= 64 * FRC ? 2 * FRC : w.sort_n >= 16 * FRC ? FRC : 0;
+	if (f) w.FL0 = w.sort_lds, w.FL1 = w.sort_lds + f, w.frc = f;
 }
 
 __host__ __device__ inline size_t ws_elems(const Caps &c) {
