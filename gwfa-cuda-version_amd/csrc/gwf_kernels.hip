@@ -744,10 +744,12 @@ __device__ __forceinline__ bool mf_next(QP Q, i32 n_s, XP X, i32 n_x, HP Hl, HP 
 // diagonal and tests the history (coalesced intervals Hl/Hh, nh of them)
 // with a pointer that only moves forward.  Kept states go to P: straight from
 // registers, in chunks of 8 steps (see below).  Returns their count.
-// QP, XP and HP are lds_cptr or const u64 * (where each list lives).
+// QP, XP and HP are lds_cptr or const u64 * (where each list lives).  The
+// kept states go to P[p_base..); prev0 is the diagonal of the state merged
+// just before Q[0] and X[0] (~0: none).
 template <int NW, typename QP, typename XP, typename HP>
 __device__ __forceinline__ i32 merge_filter(QP Q, i32 n_s, XP X, i32 n_x, HP Hl, HP Hh, i32 nh, u64 *M,
-                                            u64 *P, const Blk<NW> &b) {
+                                            u64 *P, const Blk<NW> &b, i32 p_base = 0, u64 prev0 = ~0ull) {
 	constexpr i32 NT = Blk<NW>::NT;
 	constexpr i32 CH = 8;
 	const i32 n_m = n_s + n_x;
@@ -755,7 +757,7 @@ __device__ __forceinline__ i32 merge_filter(QP Q, i32 n_s, XP X, i32 n_x, HP Hl,
 	const i32 o0 = (i32)b.tid() * E;
 	const i32 o1 = o0 + E < n_m ? o0 + E : n_m;
 	i32 i = 0, jx = 0, h = 0;
-	u64 prev_dg = ~0ull, qh = EMPTY_KEY, xh = EMPTY_KEY, hl = EMPTY_KEY, hh = EMPTY_KEY;
+	u64 prev_dg = prev0, qh = EMPTY_KEY, xh = EMPTY_KEY, hl = EMPTY_KEY, hh = EMPTY_KEY;
 	if (o0 < n_m) {
 		i32 lo = o0 - n_x > 0 ? o0 - n_x : 0, up = o0 < n_s ? o0 : n_s;
 		while (lo < up) {
@@ -804,7 +806,7 @@ __device__ __forceinline__ i32 merge_filter(QP Q, i32 n_s, XP X, i32 n_x, HP Hl,
 			kv[u] = x;
 		}
 	}
-	const i32 at = b.scan(in_m + __popc(km), &n_new);
+	const i32 at = p_base + b.scan(in_m + __popc(km), &n_new);
 	for (i32 e0 = 0; e0 < in_m; e0 += 4) {
 		u64 t[4];
 #pragma unroll
@@ -1700,7 +1702,26 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 				const lu64 *Hl = Hl_l, *Hh = Hh_l;
 				if (q_lds && x_lds) n_new = merge_filter<NW>((lds_cptr)Ql, n_s, Xl, n_x, Hl, Hh, nh2, M, w.P, b);
 				else if (q_lds) n_new = merge_filter<NW>((lds_cptr)Ql, n_s, Xg, n_x, Hl, Hh, nh2, M, w.P, b);
-				else if (x_lds) n_new = merge_filter<NW>((const u64 *)w.Q, n_s, Xl, n_x, Hl, Hh, nh2, M, w.P, b);
+				else if (x_lds && avail - used >= 256) {
+					// Q did not fit: staged window by window (Q[qa, qb) and the X
+					// entries before Q[qb] in merged order), each merged from LDS
+					const i32 QW = avail - used;
+					lu64 *qw = w.sort_lds + used;
+					i32 qa = 0, xa = 0;
+					u64 prev = ~0ull;
+					while (qa < n_s || xa < n_x) {
+						const i32 qb = qa + QW < n_s ? qa + QW : n_s;
+						const i32 xb = qb < n_s ? xa + lower_bound(Xl + xa, n_x - xa, w.Q[qb]) : n_x;
+						stage(qw, w.Q + qa, qb - qa, b);
+						b.sync();
+						n_new += merge_filter<NW>((lds_cptr)qw, qb - qa, Xl + xa, xb - xa, Hl, Hh, nh2, M, w.P, b, n_new,
+						                          prev);
+						// the last state merged: the larger of the two lists' last ones
+						const u64 lq = qb > qa ? w.Q[qb - 1] : 0, lx = xb > xa ? Xl[xb - 1] : 0;
+						if (qb > qa || xb > xa) prev = key_diag(lq > lx ? lq : lx);
+						qa = qb, xa = xb;
+					}
+				} else if (x_lds) n_new = merge_filter<NW>((const u64 *)w.Q, n_s, Xl, n_x, Hl, Hh, nh2, M, w.P, b);
 				else n_new = merge_filter<NW>((const u64 *)w.Q, n_s, Xg, n_x, Hl, Hh, nh2, M, w.P, b);
 			} else {  // the history did not fit in LDS: every list through generic pointers
 				const u64 *Qg = q_lds ? (const u64 *)Ql : (const u64 *)w.Q;
