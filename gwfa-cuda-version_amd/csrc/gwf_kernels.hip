@@ -1644,6 +1644,14 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 	i32 lds_used = 0;    // HBM mode: elements of the LDS scratch holding the sorted X
 	bool x_lds = !HBM;   // the sorted X is in LDS
 	if constexpr (HBM) {
+		// X sorted in LDS is worth more than Q or the history staged there:
+		// give up their copies (the merge stages them again if room is left)
+		// when that lets X fit
+		if (2 * n_x > w.sort_n - hist_res - q_res && 2 * n_x <= w.sort_n - hist_res) q_res = 0;
+		if (2 * n_x > w.sort_n - hist_res - q_res && 2 * n_x <= w.sort_n) {
+			hist_res = 0, q_res = 0;
+			Hl_l = Hh_l = nullptr;
+		}
 		const i32 free_n = w.sort_n - hist_res - q_res;  // LDS scratch below the staged history and Q
 		i32 free_cap = 128;
 		while (free_cap * 2 <= free_n) free_cap <<= 1;
