@@ -681,6 +681,27 @@ __device__ __forceinline__ bool forbidden(PH hlo, PH hhi, i32 nh, u64 dg) {
 	return p >= 0 && dg < hhi[p];
 }
 
+// Drop the states of X[0..n) whose diagonal is forbidden (in place, order
+// kept); returns the count left.  Such a state is never kept by the filter,
+// and neither is any other state of its diagonal, so the result is the same.
+// Each chunk is read whole before any of it is written (the block scan's
+// barrier), and a kept state moves only down, below the next chunk.
+template <int NW, class PX, class PH>
+__device__ __forceinline__ i32 drop_forbidden(PX X, i32 n, PH hlo, PH hhi, i32 nh, const Blk<NW> &b) {
+	constexpr i32 NT = Blk<NW>::NT;
+	i32 nk = 0;
+	for (i32 base = 0; base < n; base += NT) {
+		const i32 j = base + (i32)b.tid();
+		const bool act = j < n;
+		const u64 x = act ? X[j] : 0;
+		const bool keep = act && !forbidden(hlo, hhi, nh, key_diag(x));
+		const i32 sl = b.slot(keep, &nk);
+		if (keep) X[sl] = x;
+	}
+	b.sync();
+	return nk;
+}
+
 // LDS-qualified view of a pointer into the arena: loads through it compile
 // to LDS instructions instead of flat ones.
 typedef const __attribute__((address_space(3))) u64 *lds_cptr;
@@ -1644,6 +1665,11 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 	i32 lds_used = 0;    // HBM mode: elements of the LDS scratch holding the sorted X
 	bool x_lds = !HBM;   // the sorted X is in LDS
 	if constexpr (HBM) {
+		// X too large for the free scratch: first drop its forbidden states
+		if (2 * n_x > w.sort_n - hist_res - q_res) {
+			if (Hl_l) n_x = drop_forbidden(w.X, n_x, (lds_cptr)Hl_l, (lds_cptr)Hh_l, nh2, b);
+			else n_x = drop_forbidden(w.X, n_x, (const u64 *)Hlo, (const u64 *)Hhi, nh2, b);
+		}
 		// X sorted in LDS is worth more than Q or the history staged there:
 		// give up their copies (the merge stages them again if room is left)
 		// when that lets X fit
