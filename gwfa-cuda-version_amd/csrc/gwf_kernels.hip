@@ -626,20 +626,39 @@ __device__ __forceinline__ void merge_intervals_ranges(PA alo, PA ahi, i32 na, P
 }
 
 // Runs of consecutive ids in the sorted, duplicate-free T[0..n) as half-open
-// intervals [lo, hi); returns their count.
+// intervals [lo, hi); returns their count.  Each thread takes K consecutive
+// ids per pass (their loads and its neighbours' in flight together), so a
+// pass needs one block scan for K * NT ids.
 template <class B, class PT, class PO>
 __device__ __forceinline__ i32 ti_runs(PT T, i32 n, PO lo, PO hi, const B &b) {
+	constexpr i32 K = 4;
 	i32 n_ti = 0;
-	for (i32 base = 0; base < n; base += B::NT) {
-		const i32 j = base + (i32)b.tid();
-		const bool act = j < n;
-		const u64 x = act ? T[j] : 0;
-		const bool start = act && (j == 0 || T[j - 1] + 1 != x);
-		const bool end = act && (j + 1 == n || T[j + 1] != x + 1);
+	for (i32 base = 0; base < n; base += K * B::NT) {
+		const i32 j0 = base + (i32)b.tid() * K;
+		u64 x[K + 2];  // T[j0 - 1 .. j0 + K]
+#pragma unroll
+		for (i32 i = 0; i < K + 2; ++i) {
+			const i32 j = j0 - 1 + i;
+			x[i] = j >= 0 && j < n ? T[j] : 0;
+		}
+		u32 sm = 0, em = 0;  // run starts / ends among the thread's K ids
+#pragma unroll
+		for (i32 i = 0; i < K; ++i) {
+			const i32 j = j0 + i;
+			if (j < n) {
+				if (j == 0 || x[i] + 1 != x[i + 1]) sm |= 1u << i;
+				if (j + 1 == n || x[i + 2] != x[i + 1] + 1) em |= 1u << i;
+			}
+		}
 		i32 cnt;
-		const i32 e = b.scan(start ? 1 : 0, &cnt);
-		if (start) lo[n_ti + e] = x;
-		if (end) hi[n_ti + e + (start ? 1 : 0) - 1] = x + 1;
+		// index of the run holding the thread's current id (before its first
+		// start: the run begun by an earlier thread)
+		i32 r = n_ti + b.scan(__popc(sm), &cnt) - 1;
+#pragma unroll
+		for (i32 i = 0; i < K; ++i) {
+			if ((sm >> i) & 1) lo[++r] = x[i + 1];
+			if ((em >> i) & 1) hi[r] = x[i + 1] + 1;
+		}
 		n_ti += cnt;
 	}
 	return n_ti;
