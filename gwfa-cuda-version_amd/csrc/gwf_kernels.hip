@@ -669,24 +669,41 @@ __device__ __forceinline__ i32 ti_runs(PT T, i32 n, PO lo, PO hi, const B &b) {
 // caller guarantees room for n).
 template <class B, class PI, class PO>
 __device__ __forceinline__ i32 coalesce(PI lo, PI hi, i32 n, PO olo, PO ohi, const B &b) {
-	u64 carry = 0;  // max hi over previous chunks
-	i32 ng = 0;     // groups started so far
-	for (i32 base = 0; base < n; base += B::NT) {
-		const i32 j = base + (i32)b.tid();
-		const bool act = j < n;
-		const u64 l = act ? lo[j] : EMPTY_KEY;
-		const u64 h = act ? hi[j] : 0;
+	constexpr i32 K = 4;  // consecutive intervals per thread per pass
+	u64 carry = 0;        // max hi over previous passes
+	i32 ng = 0;           // groups started so far
+	for (i32 base = 0; base < n; base += K * B::NT) {
+		const i32 j0 = base + (i32)b.tid() * K;
+		u64 l[K + 1], h[K];
+#pragma unroll
+		for (i32 i = 0; i < K; ++i) {
+			const i32 j = j0 + i;
+			l[i] = j < n ? lo[j] : EMPTY_KEY;
+			h[i] = j < n ? hi[j] : 0;
+		}
+		l[K] = j0 + K < n ? lo[j0 + K] : EMPTY_KEY;
+		u64 sm = h[0];
+#pragma unroll
+		for (i32 i = 1; i < K; ++i) sm = sm > h[i] ? sm : h[i];
 		u64 ex, tmax;
-		u64 pin = b.scan_max2(h, &ex, &tmax);
-		pin = pin > carry ? pin : carry;
-		const u64 pex = ex > carry ? ex : carry;
-		const bool start = act && (j == 0 || l > pex);
-		const u64 nl = (act && j + 1 < n) ? lo[j + 1] : EMPTY_KEY;
-		const bool end = act && (j + 1 == n || nl > pin);
+		b.scan_max2(sm, &ex, &tmax);
+		u64 run = ex > carry ? ex : carry;  // max hi before the thread's first interval
+		u32 stm = 0, enm = 0;               // group starts / ends among its K
+#pragma unroll
+		for (i32 i = 0; i < K; ++i) {
+			const i32 j = j0 + i;
+			if (j < n && (j == 0 || l[i] > run)) stm |= 1u << i;
+			run = run > h[i] ? run : h[i];
+			h[i] = run;  // the group's hi so far
+			if (j < n && (j + 1 == n || l[i + 1] > run)) enm |= 1u << i;
+		}
 		i32 cnt;
-		const i32 e = b.scan(start ? 1 : 0, &cnt);
-		if (start) olo[ng + e] = l;
-		if (end) ohi[ng + e + (start ? 1 : 0) - 1] = pin;
+		i32 r = ng + b.scan(__popc(stm), &cnt) - 1;
+#pragma unroll
+		for (i32 i = 0; i < K; ++i) {
+			if ((stm >> i) & 1) olo[++r] = l[i];
+			if ((enm >> i) & 1) ohi[r] = h[i];
+		}
 		ng += cnt;
 		carry = carry > tmax ? carry : tmax;
 	}
