@@ -1689,10 +1689,33 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 		E *TIlo = w.P, *TIhi = w.P + n_tt;
 		const i32 n_ti = ti_runs(w.T, n_tt, TIlo, TIhi, b);
 		b.sync();
-		E *Clo = w.P + 2 * n_tt, *Chi = Clo + (n_h + n_ti);
-		if constexpr (HBM) merge_intervals_ranges(Hcur_lo, Hcur_hi, n_h, TIlo, TIhi, n_ti, Clo, Chi, b);
-		else merge_intervals2(Hcur_lo, Hcur_hi, n_h, TIlo, TIhi, n_ti, Clo, Chi, b);
-		nh2 = coalesce(Clo, Chi, n_h + n_ti, Hlo, Hhi, b);
+		if constexpr (HBM) {
+			// the exits form few runs: the merge and coalesce fit the LDS
+			// scratch even where the exits did not (current history and runs
+			// at the bottom, merged list above them, the new history at the top)
+			const i32 cap_r = n_h + n_ti;
+			if (2 * n_h + 2 * n_ti + 4 * cap_r <= w.sort_n) {
+				lu64 *hc = w.sort_lds, *rr = hc + 2 * n_h, *Clo = rr + 2 * n_ti, *Chi = Clo + cap_r;
+				lu64 *Olo = w.sort_lds + w.sort_n - 2 * cap_r, *Ohi = Olo + cap_r;
+				stage(hc, Hcur_lo, n_h, b);
+				stage(hc + n_h, Hcur_hi, n_h, b);
+				stage(rr, TIlo, n_ti, b);
+				stage(rr + n_ti, TIhi, n_ti, b);
+				b.sync();
+				merge_intervals2(hc, hc + n_h, n_h, rr, rr + n_ti, n_ti, Clo, Chi, b);
+				nh2 = coalesce(Clo, Chi, cap_r, Olo, Ohi, b);
+				for (i32 t = tid; t < nh2; t += NT) Hlo[t] = Olo[t], Hhi[t] = Ohi[t];
+				hist_res = 2 * cap_r;
+				Hl_l = Olo, Hh_l = Ohi;
+				p5_done = true;
+			}
+		}
+		if (!p5_done) {
+			E *Clo = w.P + 2 * n_tt, *Chi = Clo + (n_h + n_ti);
+			if constexpr (HBM) merge_intervals_ranges(Hcur_lo, Hcur_hi, n_h, TIlo, TIhi, n_ti, Clo, Chi, b);
+			else merge_intervals2(Hcur_lo, Hcur_hi, n_h, TIlo, TIhi, n_ti, Clo, Chi, b);
+			nh2 = coalesce(Clo, Chi, n_h + n_ti, Hlo, Hhi, b);
+		}
 	}
 
 	// ---- P4: sort the closure successors: in LDS (Xl), or in HBM (Xg, HBM mode)
