@@ -629,9 +629,8 @@ __device__ __forceinline__ void merge_intervals_ranges(PA alo, PA ahi, i32 na, P
 // intervals [lo, hi); returns their count.  Each thread takes K consecutive
 // ids per pass (their loads and its neighbours' in flight together), so a
 // pass needs one block scan for K * NT ids.
-template <class B, class PT, class PO>
-__device__ __forceinline__ i32 ti_runs(PT T, i32 n, PO lo, PO hi, const B &b) {
-	constexpr i32 K = 4;
+template <i32 K, class B, class PT, class PO>
+__device__ __forceinline__ i32 ti_runs_k(PT T, i32 n, PO lo, PO hi, const B &b) {
 	i32 n_ti = 0;
 	for (i32 base = 0; base < n; base += K * B::NT) {
 		const i32 j0 = base + (i32)b.tid() * K;
@@ -663,15 +662,20 @@ __device__ __forceinline__ i32 ti_runs(PT T, i32 n, PO lo, PO hi, const B &b) {
 	}
 	return n_ti;
 }
+// (one id per thread where one pass holds them all)
+template <class B, class PT, class PO>
+__device__ __forceinline__ i32 ti_runs(PT T, i32 n, PO lo, PO hi, const B &b) {
+	return n <= B::NT ? ti_runs_k<1>(T, n, lo, hi, b) : ti_runs_k<4>(T, n, lo, hi, b);
+}
 
 // Coalesce overlapping or adjacent intervals of a list sorted by lo
 // (gwf_intv_merge_adj, gwf-ed.cu:266-296).  Returns the output count (the
 // caller guarantees room for n).
-template <class B, class PI, class PO>
-__device__ __forceinline__ i32 coalesce(PI lo, PI hi, i32 n, PO olo, PO ohi, const B &b) {
-	constexpr i32 K = 4;  // consecutive intervals per thread per pass
-	u64 carry = 0;        // max hi over previous passes
-	i32 ng = 0;           // groups started so far
+template <i32 K, class B, class PI, class PO>
+__device__ __forceinline__ i32 coalesce_k(PI lo, PI hi, i32 n, PO olo, PO ohi, const B &b) {
+	// (K consecutive intervals per thread per pass)
+	u64 carry = 0;  // max hi over previous passes
+	i32 ng = 0;     // groups started so far
 	for (i32 base = 0; base < n; base += K * B::NT) {
 		const i32 j0 = base + (i32)b.tid() * K;
 		u64 l[K + 1], h[K];
@@ -709,6 +713,10 @@ __device__ __forceinline__ i32 coalesce(PI lo, PI hi, i32 n, PO olo, PO ohi, con
 	}
 	b.sync();
 	return ng;
+}
+template <class B, class PI, class PO>
+__device__ __forceinline__ i32 coalesce(PI lo, PI hi, i32 n, PO olo, PO ohi, const B &b) {
+	return n <= B::NT ? coalesce_k<1>(lo, hi, n, olo, ohi, b) : coalesce_k<4>(lo, hi, n, olo, ohi, b);
 }
 
 template <class PH>
