@@ -909,6 +909,52 @@ __device__ __forceinline__ bool vis_insert(lu64 *vis, u32 vmask, u64 key) {
 	return true;
 }
 
+// The claims of one arc group (key[u] for the bits u of `live`): their first
+// probes are issued together, the rare collisions probed on one key at a time
+// (as vis_insert).  Returns the mask of the keys that were absent (and are now
+// present; a full probe window reports "absent").
+__device__ __forceinline__ u32 vis_insert4(lu64 *vis, u32 vmask, const u64 (&key)[4], u32 live) {
+	u32 h[4];
+	u64 old[4];
+#pragma unroll
+	for (int u = 0; u < 4; ++u) {
+		h[u] = vis_hash(key[u]) & vmask;
+		old[u] = EMPTY_KEY;
+		if ((live >> u) & 1)
+			__hip_atomic_compare_exchange_strong(&vis[h[u]], &old[u], key[u], __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+			                                     __HIP_MEMORY_SCOPE_WORKGROUP);
+	}
+	u32 absent = 0, pend = 0;
+#pragma unroll
+	for (int u = 0; u < 4; ++u)
+		if ((live >> u) & 1) {
+			if (old[u] == EMPTY_KEY) absent |= 1u << u;
+			else if (old[u] != key[u]) pend |= 1u << u;
+		}
+	if (__ballot(pend != 0)) {
+#pragma unroll
+		for (int u = 0; u < 4; ++u)
+			if ((pend >> u) & 1) {
+				u32 hh = h[u];
+				bool res = true;
+#pragma unroll 1
+				for (int p = 1; p < 16; ++p) {
+					hh = (hh + 1) & vmask;
+					u64 o = EMPTY_KEY;
+					if (__hip_atomic_compare_exchange_strong(&vis[hh], &o, key[u], __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+					                                         __HIP_MEMORY_SCOPE_WORKGROUP))
+						break;
+					if (o == key[u]) {
+						res = false;
+						break;
+					}
+				}
+				if (res) absent |= 1u << u;
+			}
+	}
+	return absent;
+}
+
 // Slot of key K in a read's visited set (w.VK/VS/vmask), inserted when absent
 // if `insert`; -1 when the probe window is exhausted (the set counts as full).
 template <class BF>
@@ -1261,16 +1307,25 @@ __device__ __forceinline__ int closure(const DGraph &g, const uint8_t *q, i32 ql
 					const uint2 t = *(const uint2 *)&g.arc[has ? vi.z + (u32)(e0 + u) : 0u];
 					ar[u] = has ? t : make_uint2(0, 0);
 				}
-				u32 pa_m = 0, px_m = 0, pq_m = 0;
+				u32 pa_m = 0, px_m = 0, pq_m = 0, live = 0;
 				i32 ca = 0, cx = 0;
+				u64 vk[AG];
+#pragma unroll
+				for (i32 u = 0; u < AG; ++u) {
+					const bool has = e0 + u < deg;
+					vk[u] = (u64)ar[u].x << 32 | (u32)(i + 1);
+					live |= (has && !qe ? 1u : 0u) << u;
+				}
+				static_assert(AG == 4, "one visited-set call per arc group");
+				const u32 absent = vis_insert4(vis, vmask, vk, live);
 #pragma unroll
 				for (i32 u = 0; u < AG; ++u) {
 					const bool has = e0 + u < deg;
 					const bool m = has && !qe && (uint8_t)(ar[u].y >> 24) == c;
 					n_ext += m;
-					const bool absent = has && !qe && vis_insert(vis, vmask, (u64)ar[u].x << 32 | (u32)(i + 1));
-					pa_m |= (absent && m ? 1u : 0u) << u;
-					px_m |= (absent && !m ? 1u : 0u) << u;
+					const bool ab = (absent >> u) & 1;
+					pa_m |= (ab && m ? 1u : 0u) << u;
+					px_m |= (ab && !m ? 1u : 0u) << u;
 					pq_m |= (has && qe ? 1u : 0u) << u;
 				}
 				ca = __popc(pa_m);
