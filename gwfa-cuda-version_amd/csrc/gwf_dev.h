@@ -47,10 +47,13 @@ inline int gwf_pick_layout(uint64_t n_vtx, u32 max_vlen, i32 max_ql) {
 #define GWF_MAX_VLEN ((1 << 21) - 1)
 #define GWF_MAX_QLEN ((1 << 21) - 2)
 
-// Device graph (packed CSR).  vinfo[v]: x = byte offset of v's sequence in
-// `seq`, y = length, z = first out-arc, w = out-degree.  arc[j] carries what
-// crossing the arc needs in one 16-byte load: x = target w, y = overlap o |
-// (base w[o] << 24), z = w's sequence offset, w = w's length.
+// Device graph (packed CSR).  Two 16-byte records per vertex: vinfo[2 v]:
+// x = byte offset of v's sequence in `seq`, y = length, z = first out-arc,
+// w = out-degree; vinfo[2 v + 1]: the first two out-arcs' (x, y) as in arc[]
+// (zero where v has fewer), so the closure crosses most vertices without a
+// second dependent load.  arc[j] carries what crossing the arc needs in one
+// 16-byte load: x = target w, y = overlap o | (base w[o] << 24), z = w's
+// sequence offset, w = w's length.
 struct DGraph {
 	const uint4 *vinfo;
 	const uint8_t *seq;

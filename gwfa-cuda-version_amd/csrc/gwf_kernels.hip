@@ -1166,7 +1166,7 @@ __device__ __forceinline__ bool closure_mixed(const DGraph &g, const uint8_t *q,
 		const bool act = j < n_f;
 		const u64 key = act ? front_get(w, w.F0, w.FL0, j) : 0;
 		const i32 i = key_d(key) + key_k(key);
-		const uint4 vi = act ? g.vinfo[key_v(key)] : make_uint4(0, 0, 0, 0);
+		const uint4 vi = act ? g.vinfo[2 * (size_t)key_v(key)] : make_uint4(0, 0, 0, 0);
 		const i32 deg = act && i + 1 != ql ? (i32)vi.w : 0;  // a query end claims no key
 		i32 tot;
 		const i32 r0 = rank_base + b.scan(deg, &tot);
@@ -1188,7 +1188,7 @@ __device__ __forceinline__ bool closure_mixed(const DGraph &g, const uint8_t *q,
 		const u64 key = act ? front_get(w, w.F0, w.FL0, j) : 0;
 		const u32 v = key_v(key);
 		const i32 d = key_d(key), k = key_k(key), i = d + k;
-		const uint4 vi = act ? g.vinfo[v] : make_uint4(0, 0, 0, 0);
+		const uint4 vi = act ? g.vinfo[2 * (size_t)v] : make_uint4(0, 0, 0, 0);
 		const bool rec = act && closure_origin;
 		// query end at a vertex end (only with v1 >= 0): delete into every next
 		// vertex, nothing else (gwf-ed.cu:1244-1254)
@@ -1285,7 +1285,9 @@ __device__ __forceinline__ int closure(const DGraph &g, const uint8_t *q, i32 ql
 			const u32 v = key_v(key);
 			const i32 d = key_d(key), k = key_k(key);
 			const i32 i = d + k;
-			const uint4 vi = act ? g.vinfo[v] : make_uint4(0, 0, 0, 0);
+			// the vertex record and, beside it, its first two out-arcs
+			const uint4 vi = act ? g.vinfo[2 * (size_t)v] : make_uint4(0, 0, 0, 0);
+			const uint4 va = act ? g.vinfo[2 * (size_t)v + 1] : make_uint4(0, 0, 0, 0);
 			const bool rec = act && closure_origin;
 			// query end at a vertex end (only with v1 >= 0): delete into every
 			// next vertex, nothing else (gwf-ed.cu:1244-1254)
@@ -1297,15 +1299,30 @@ __device__ __forceinline__ int closure(const DGraph &g, const uint8_t *q, i32 ql
 			// arcs in groups of AG: the group's arc records are loaded together
 			// (from a valid address when the lane has no arc there: loads under
 			// a branch are waited for one after the other), its claims made, and
-			// its successors placed with one scan
+			// its successors placed with one scan.  The first two come with the
+			// vertex record; arcs 2 and 3 are loaded only where a vertex has them.
 			constexpr i32 AG = 4;
 			for (i32 e0 = 0; e0 < maxdeg; e0 += AG) {
 				uint2 ar[AG];
+				if (e0 == 0) {
+					ar[0] = 0 < deg ? make_uint2(va.x, va.y) : make_uint2(0, 0);
+					ar[1] = 1 < deg ? make_uint2(va.z, va.w) : make_uint2(0, 0);
+					ar[2] = ar[3] = make_uint2(0, 0);
+					if (maxdeg > 2) {
 #pragma unroll
-				for (i32 u = 0; u < AG; ++u) {
-					const bool has = e0 + u < deg;
-					const uint2 t = *(const uint2 *)&g.arc[has ? vi.z + (u32)(e0 + u) : 0u];
-					ar[u] = has ? t : make_uint2(0, 0);
+						for (i32 u = 2; u < AG; ++u) {
+							const bool has = u < deg;
+							const uint2 t = *(const uint2 *)&g.arc[has ? vi.z + (u32)u : 0u];
+							ar[u] = has ? t : make_uint2(0, 0);
+						}
+					}
+				} else {
+#pragma unroll
+					for (i32 u = 0; u < AG; ++u) {
+						const bool has = e0 + u < deg;
+						const uint2 t = *(const uint2 *)&g.arc[has ? vi.z + (u32)(e0 + u) : 0u];
+						ar[u] = has ? t : make_uint2(0, 0);
+					}
 				}
 				u32 pa_m = 0, px_m = 0, pq_m = 0, live = 0;
 				i32 ca = 0, cx = 0;
@@ -1377,7 +1394,7 @@ __device__ __forceinline__ int closure(const DGraph &g, const uint8_t *q, i32 ql
 			const u32 v = key_v(key);
 			const i32 d = key_d(key);
 			i32 k = key_k(key);
-			const uint4 vi = act ? g.vinfo[v] : make_uint4(0, 0, 0, 0);
+			const uint4 vi = act ? g.vinfo[2 * (size_t)v] : make_uint4(0, 0, 0, 0);
 			const i32 vl = (i32)vi.y;
 			const i32 max_k = (ql - d < vl ? ql - d : vl) - 1;
 			k = extend_wave(act, g.seq, vi.x, q, d, k, max_k, &chars);
@@ -1503,7 +1520,7 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 			// state): loads under a branch are waited for one after the other
 #pragma unroll
 			for (i32 s = 0; s < S; ++s) {
-				const uint2 t = *(const uint2 *)&g.vinfo[key[s] != EMPTY_KEY ? key_v(key[s]) : 0u];
+				const uint2 t = *(const uint2 *)&g.vinfo[key[s] != EMPTY_KEY ? 2 * (size_t)key_v(key[s]) : 0];
 				vi[s] = key[s] != EMPTY_KEY ? t : make_uint2(0, 0);
 			}
 			i32 kk[S], mk8[S];

@@ -376,7 +376,7 @@ int gwf_dgraph_upload(const gwf_graph_t *g, gwf_dgraph_t **out) {
 	int rc = device_ready();
 	if (rc) return rc;
 	const u32 nv = g->n_vtx, na = g->n_arc;
-	std::vector<uint4> vinfo(nv);
+	std::vector<uint4> vinfo(2 * (size_t)nv);
 	std::vector<i32> in_ol(nv, -1);
 	bool mixed = false;
 	uint64_t total = 0;
@@ -419,10 +419,13 @@ int gwf_dgraph_upload(const gwf_graph_t *g, gwf_dgraph_t **out) {
 	uint64_t so = 0;
 	for (u32 v = 0; v < nv; ++v) {
 		const uint64_t ax = g->aux ? g->aux[v] : 0;
-		vinfo[v] = make_uint4((u32)so, g->len[v], (u32)(ax >> 32), (u32)ax);
+		const u32 a0 = (u32)(ax >> 32), deg = (u32)ax;
+		vinfo[2 * (size_t)v] = make_uint4((u32)so, g->len[v], a0, deg);
+		vinfo[2 * (size_t)v + 1] = make_uint4(deg > 0 ? arcs[a0].x : 0, deg > 0 ? arcs[a0].y : 0,
+		                                      deg > 1 ? arcs[a0 + 1].x : 0, deg > 1 ? arcs[a0 + 1].y : 0);
 		so += g->len[v];
 	}
-	const size_t b_vi = sizeof(uint4) * nv, b_seq = (size_t)total + kSeqPad, b_arc = sizeof(uint4) * arcs.size();
+	const size_t b_vi = sizeof(uint4) * vinfo.size(), b_seq = (size_t)total + kSeqPad, b_arc = sizeof(uint4) * arcs.size();
 	auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
 	const size_t bytes = al(b_vi) + al(b_seq) + al(b_arc);
 	std::vector<uint8_t> seqh(b_seq, 0);
