@@ -93,9 +93,11 @@ __device__ __forceinline__ u32 rank_below(u64 mask) {
 #define DPP_WAVE_SHR1 0x138
 #define DPP_WAVE_SHL1 0x130
 
+// (with full row masks a lane whose source is out of range reads 0, the old
+// value every caller passes, and the destination needs no copy of it)
 template <int CTRL, int ROWS>
 __device__ __forceinline__ i32 dpp_i32(i32 old, i32 v) {
-	return __builtin_amdgcn_update_dpp(old, v, CTRL, ROWS, 0xf, false);
+	return __builtin_amdgcn_update_dpp(old, v, CTRL, ROWS, 0xf, ROWS == 0xf);
 }
 template <int CTRL, int ROWS>
 __device__ __forceinline__ u64 dpp_u64(u64 v) {
