@@ -52,8 +52,11 @@ inline int gwf_pick_layout(uint64_t n_vtx, u32 max_vlen, i32 max_ql) {
 // w = out-degree; vinfo[2 v + 1]: the first two out-arcs' (x, y) as in arc[]
 // (zero where v has fewer), so the closure crosses most vertices without a
 // second dependent load.  arc[j] carries what crossing the arc needs in one
-// 16-byte load: x = target w, y = overlap o | (base w[o] << 24), z = w's
-// sequence offset, w = w's length.
+// 16-byte load: x = target w, y = overlap o | GWF_ARC_SINGLE_IN when this arc
+// is w's only in-arc | (base w[o] << 24), z = w's sequence offset, w = w's
+// length.  (o < 2^21, below the flag bit.)
+#define GWF_ARC_SINGLE_IN (1u << 23)
+#define GWF_ARC_OL_MASK 0x7FFFFFu
 struct DGraph {
 	const uint4 *vinfo;
 	const uint8_t *seq;

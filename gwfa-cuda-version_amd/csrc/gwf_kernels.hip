@@ -1230,7 +1230,7 @@ __device__ __forceinline__ bool closure_mixed(const DGraph &g, const uint8_t *q,
 		for (i32 e = 0; e < degall; ++e) {
 			const uint4 ar = g.arc[vi.z + e];
 			const u32 wv = ar.x;
-			const i32 ol = (i32)(ar.y & 0xFFFFFFu);
+			const i32 ol = (i32)(ar.y & GWF_ARC_OL_MASK);
 			if (qe) {
 				if (sx < cap.x) w.X[sx] = mk(wv, i - ol, ol);
 				++sx;
@@ -1326,17 +1326,23 @@ __device__ __forceinline__ int closure(const DGraph &g, const uint8_t *q, i32 ql
 						ar[u] = has ? t : make_uint2(0, 0);
 					}
 				}
-				u32 pa_m = 0, px_m = 0, pq_m = 0, live = 0;
+				// an arc that is its target's only in-arc claims nothing: the key
+				// (target, i + 1) can repeat only where this state does, and the
+				// copies it would make are removed with the other duplicates of a
+				// diagonal by the merge
+				u32 pa_m = 0, px_m = 0, pq_m = 0, live = 0, single = 0;
 				i32 ca = 0, cx = 0;
 				u64 vk[AG];
 #pragma unroll
 				for (i32 u = 0; u < AG; ++u) {
-					const bool has = e0 + u < deg;
+					const bool cl = e0 + u < deg && !qe;
+					const bool one = (ar[u].y & GWF_ARC_SINGLE_IN) != 0;
 					vk[u] = (u64)ar[u].x << 32 | (u32)(i + 1);
-					live |= (has && !qe ? 1u : 0u) << u;
+					live |= (cl && !one ? 1u : 0u) << u;
+					single |= (cl && one ? 1u : 0u) << u;
 				}
 				static_assert(AG == 4, "one visited-set call per arc group");
-				const u32 absent = vis_insert4(vis, vmask, vk, live);
+				const u32 absent = vis_insert4(vis, vmask, vk, live) | single;
 #pragma unroll
 				for (i32 u = 0; u < AG; ++u) {
 					const bool has = e0 + u < deg;
@@ -1357,7 +1363,7 @@ __device__ __forceinline__ int closure(const DGraph &g, const uint8_t *q, i32 ql
 #pragma unroll
 				for (i32 u = 0; u < AG; ++u) {
 					const u32 wv = ar[u].x;
-					const i32 ol = (i32)(ar[u].y & 0xFFFFFFu);
+					const i32 ol = (i32)(ar[u].y & GWF_ARC_OL_MASK);
 					if ((pa_m >> u) & 1) {
 						if (sa < cap.f) front_put(w, w.F1, w.FL1, sa, mk(wv, i + 1 - ol, ol));
 						++sa;

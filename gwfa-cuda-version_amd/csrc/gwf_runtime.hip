@@ -400,6 +400,12 @@ int gwf_dgraph_upload(const gwf_graph_t *g, gwf_dgraph_t **out) {
 		uint64_t o = 0;
 		for (u32 v = 0; v < nv; ++v) seq_off[v] = (u32)o, o += g->len[v];
 	}
+	// in-arc counts: a target with one in-arc is flagged in its arc record
+	std::vector<u32> indeg(nv, 0);
+	for (u32 i = 0; i < na; ++i) {
+		const u32 w = (u32)g->arc[i].a;
+		if (w < nv) ++indeg[w];
+	}
 	for (u32 i = 0; i < na; ++i) {
 		const u32 u = (u32)(g->arc[i].a >> 32), w = (u32)g->arc[i].a;
 		const i32 o = g->arc[i].o;
@@ -414,7 +420,8 @@ int gwf_dgraph_upload(const gwf_graph_t *g, gwf_dgraph_t **out) {
 		// closure in that order and one visited set per read (DGraph::mixed).
 		if (in_ol[w] >= 0 && in_ol[w] != o) mixed = true;
 		in_ol[w] = o;
-		arcs[i] = make_uint4(w, (u32)o | (u32)(uint8_t)g->seq[w][o] << 24, seq_off[w], g->len[w]);
+		arcs[i] = make_uint4(w, (u32)o | (indeg[w] == 1 ? GWF_ARC_SINGLE_IN : 0u) | (u32)(uint8_t)g->seq[w][o] << 24,
+		                     seq_off[w], g->len[w]);
 	}
 	uint64_t so = 0;
 	for (u32 v = 0; v < nv; ++v) {
