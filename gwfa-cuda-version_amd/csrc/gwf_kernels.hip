@@ -896,25 +896,11 @@ __device__ __forceinline__ u32 vis_hash(u64 x) {
 	x ^= x >> 27;
 	return (u32)x;
 }
-// true if key was absent (and is now present); a full probe window reports
-// "absent", which only costs duplicate work.
-__device__ __forceinline__ bool vis_insert(lu64 *vis, u32 vmask, u64 key) {
-	u32 h = vis_hash(key) & vmask;
-	for (int p = 0; p < 16; ++p) {
-		u64 old = EMPTY_KEY;
-		if (__hip_atomic_compare_exchange_strong(&vis[h], &old, key, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-		                                         __HIP_MEMORY_SCOPE_WORKGROUP))
-			return true;
-		if (old == key) return false;
-		h = (h + 1) & vmask;
-	}
-	return true;
-}
-
-// The claims of one arc group (key[u] for the bits u of `live`): their first
-// probes are issued together, the rare collisions probed on one key at a time
-// (as vis_insert).  Returns the mask of the keys that were absent (and are now
-// present; a full probe window reports "absent").
+// The claims of one arc group (key[u] for the bits u of `live`) in a per-step
+// visited set (open addressing, linear probing over 16 positions): the first
+// probes are issued together, the rare collisions probed on one key at a time.
+// Returns the mask of the keys that were absent (and are now present; a full
+// probe window reports "absent", which only costs duplicate work).
 __device__ __forceinline__ u32 vis_insert4(lu64 *vis, u32 vmask, const u64 (&key)[4], u32 live) {
 	u32 h[4];
 	u64 old[4];
