@@ -162,6 +162,50 @@ def test_high_degree_vertices_vs_oracle(gpu, orc, seed, dag, ovl, kernel):
 
 
 @pytest.mark.parametrize("kernel", KERNELS)
+@pytest.mark.parametrize("seed,cyclic", [(31, False), (32, True)])
+def test_single_in_arc_chains_vs_oracle(gpu, orc, seed, cyclic, kernel):
+    """Diamonds whose two branches carry the same sequence (both reach the
+    merge vertex at one query position), followed by chains of vertices with
+    a single in-arc, which the closure crosses without visited-set claims;
+    optionally a back arc from the last chain vertex to the first diamond.
+    Repeated frontier states then run down the chains twice: the EDs must
+    still equal the oracle's."""
+    _say(f"case: single-in-arc chains seed={seed} cyclic={cyclic} kernel {kernel}")
+    rng = random.Random(seed)
+    for it in range(10):
+        seqs, arcs = [], []
+        prev = None
+        for _ in range(rng.randint(2, 6)):
+            split = len(seqs)
+            seqs.append("".join(rng.choice("ACGT") for _ in range(rng.randint(1, 6))))
+            br = "".join(rng.choice("ACGT") for _ in range(rng.randint(1, 5)))
+            seqs += [br, br if rng.random() < 0.8 else br[::-1]]
+            merge = len(seqs)
+            seqs.append("".join(rng.choice("ACGT") for _ in range(rng.randint(1, 6))))
+            arcs += [(split, split + 1), (split, split + 2), (split + 1, merge), (split + 2, merge)]
+            last = merge
+            for _ in range(rng.randint(1, 4)):  # single-in-arc chain
+                seqs.append("".join(rng.choice("ACGT") for _ in range(rng.randint(1, 8))))
+                arcs.append((last, len(seqs) - 1))
+                last = len(seqs) - 1
+            if prev is not None:
+                arcs.append((prev, split))
+            prev = last
+        if cyclic:
+            arcs.append((prev, 0))
+        g = gpu.Graph.from_lists(seqs, [(v, w, 0) for v, w in arcs])
+        arcs_of = {}
+        for v, w, o in g.arcs():
+            arcs_of.setdefault(v, []).append((w, o))
+        reads = [_sample(rng, seqs, arcs_of, rng.randint(1, 200), rng.choice([0.0, 0.05, 0.2])) for _ in range(24)]
+        r = gpu.Reads.from_list(reads)
+        g.index()
+        want, _, _ = orc.batch(g, r)
+        got, _ = _gpu_eds(gpu, g, r, kernel)
+        assert np.array_equal(got, want), f"iteration {it}: gpu {got.tolist()} oracle {want.tolist()}"
+
+
+@pytest.mark.parametrize("kernel", KERNELS)
 def test_edge_cases(gpu, orc, kernel):
     _say(f"case: edge cases, kernel {kernel}")
     # short start vertex (< 32 bp) read as an exact prefix: the reference's
