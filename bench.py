@@ -149,10 +149,20 @@ def serve_bench(gw, args):
         res = srv.collect(n)
         mism += int((res["ed"] != want[ids - ids[0]]).sum())
         sat_s = (int(res["t_done_ns"].max()) - t0) / 1e9
-        sat_rps = n / sat_s
-        out["saturated"] = {"reads_per_s": round(sat_rps, 1), "bases_per_s": round(int(ql.sum()) / sat_s, 1),
-                            "seconds": round(sat_s, 3), "stats": srv.stats()}
-        log(f"saturated: {sat_rps:.0f} reads/s, {ql.sum() / sat_s / 1e6:.1f} M bases/s")
+        # the capacity: completions per second while the queue is full (between
+        # the 10th and 90th percentile completion times); the makespan rate also
+        # holds the first-in-first-out tail of the longest reads
+        done = np.sort(res["t_done_ns"].astype(np.int64))
+        lo, hi = done[int(0.1 * (n - 1))], done[int(0.9 * (n - 1))]
+        sat_rps = float((int(0.9 * (n - 1)) - int(0.1 * (n - 1))) / max((hi - lo) / 1e9, 1e-9))
+        bpr = int(ql.sum()) / n
+        out["saturated"] = {"reads_per_s": round(sat_rps, 1), "bases_per_s": round(sat_rps * bpr, 1),
+                            "definition": "completions per second between the 10th and 90th percentile completion "
+                                          "times, every request released at once",
+                            "makespan_reads_per_s": round(n / sat_s, 1), "seconds": round(sat_s, 3),
+                            "stats": srv.stats()}
+        log(f"saturated: {sat_rps:.0f} reads/s ({sat_rps * bpr / 1e6:.1f} M bases/s) while the queue is full, "
+            f"{n / sat_s:.0f} reads/s over the makespan")
         rng = np.random.default_rng(5)
         sweep = []
         for f in fracs:
@@ -182,8 +192,12 @@ def serve_bench(gw, args):
         out["sweep"] = sweep
         out["stats"] = srv.stats()
     dg.free()
-    top = max(sweep, key=lambda r: r["load"])
+    # the headline: the highest offered load served without queueing (admission
+    # queue p99 under 1 ms); the loads above it show where the queue builds
+    unq = [r for r in sweep if r["queue_p99_ms"] <= 1.0]
+    top = max(unq or sweep, key=lambda r: r["load"])
     out["value"] = top["sustained_bases_per_s"]
+    out["value_rule"] = "sustained bases/s at the highest offered load whose admission queue p99 is under 1 ms"
     out["latency_ms"] = {"p50": top["p50_ms"], "p99": top["p99_ms"], "at_load": top["load"]}
     out["parity"] = {"requests_checked": int(out["stats"]["completed"]), "mismatches": mism,
                      "against": "gwf_ed_run persistent batch EDs of the same reads"}
@@ -214,7 +228,8 @@ def main():
                     help="persistent kernel: autotune the bin table on the first N reads before timing, untimed "
                          "(-1: all reads; 0: use the built-in table)")
     ap.add_argument("--serve", action="store_true", help="serving sweep (Poisson arrivals) instead of batch throughput")
-    ap.add_argument("--rates", default="0.25,0.5,0.75,0.9,1.0", help="--serve: loads as fractions of saturation")
+    ap.add_argument("--rates", default="0.25,0.5,0.75,0.9,1.0,1.25,1.5",
+                    help="--serve: loads as fractions of the saturated rate")
     ap.add_argument("--serve-seconds", type=float, default=1.5, help="--serve: arrival window per load")
     ap.add_argument("--budget", type=int, default=4096, help="--serve: device state budget, MiB")
     ap.add_argument("--serve-nw", type=int, default=0, help="--serve: waves per request (default: the server's, 4)")
