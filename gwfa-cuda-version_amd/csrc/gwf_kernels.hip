@@ -193,6 +193,20 @@ struct Blk {
 		*tc = t >> 16;
 		return p & 0xffff;
 	}
+	// three exclusive scans in one for NW <= 2 (per-thread values below 8,
+	// block totals below 2^9, 2^10, 2^10, so the packed sum stays below 2^29):
+	// *pb / *pc receive the second and third, *ta, *tb, *tc the totals
+	__device__ __forceinline__ i32 scan3(i32 a, i32 bb, i32 c, i32 *pb, i32 *pc, i32 *ta, i32 *tb, i32 *tc) const {
+		static_assert(NW <= 2, "packed fields sized for at most 128 threads");
+		i32 t;
+		const i32 p = scan(a | bb << 9 | c << 19, &t);
+		*pb = (p >> 9) & 0x3ff;
+		*pc = p >> 19;
+		*ta = t & 0x1ff;
+		*tb = (t >> 9) & 0x3ff;
+		*tc = t >> 19;
+		return p & 0x1ff;
+	}
 	// compaction slot of a predicate; *n advances by the block count
 	__device__ __forceinline__ i32 slot(bool pred, i32 *n) const {
 		const u64 m = __ballot(pred);
@@ -1497,13 +1511,19 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 	i32 n_f = 0, n_s = 0, n_t = 0;
 	u64 pk[S];   // the previous round's extended states (EMPTY_KEY past the end)
 	i32 pvl[S];  // and their vertex lengths
+	// With at most two waves a round's vertex ends go to the frontier one
+	// round later, with its successors: one scan places all three lists, and
+	// a plain barrier publishes the edge slots before the successors
+	constexpr bool ONE_SCAN = NW <= 2;
+	bool pve[S];  // the previous round's vertex ends (ONE_SCAN)
 #pragma unroll
-	for (i32 s = 0; s < S; ++s) pk[s] = EMPTY_KEY, pvl[s] = 0;
+	for (i32 s = 0; s < S; ++s) pk[s] = EMPTY_KEY, pvl[s] = 0, pve[s] = false;
 	for (i32 r = 0; r <= R; ++r) {
 		u64 nk[S];
 		i32 nvl[S];
+		bool vend[S];  // vertex ends of round r
 #pragma unroll
-		for (i32 s = 0; s < S; ++s) nk[s] = EMPTY_KEY, nvl[s] = 0;
+		for (i32 s = 0; s < S; ++s) nk[s] = EMPTY_KEY, nvl[s] = 0, vend[s] = false;
 		if (r < R) {  // extend round r
 			const i32 j0 = r * RS + tid * S;
 			u64 key[S];
@@ -1544,7 +1564,6 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 #pragma unroll
 			for (i32 s = 0; s < S; ++s)  // every thread calls it S times (it is a wave collective)
 				kk[s] = extend_wave(more[s], g.seq, vi[s].x, q, key_d(key[s]), kk[s], mk8[s], &chars);
-			bool vend[S];
 			i32 nv = 0;
 #pragma unroll
 			for (i32 s = 0; s < S; ++s) {
@@ -1568,17 +1587,22 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 			}
 			if (lane == 0) edgeF[(r % 3) * NW + wv] = nk[0];
 			if (lane == WAVE - 1) edgeL[(r % 3) * NW + wv] = nk[S - 1];
-			i32 tot;
-			i32 pos = n_f + b.scan(nv, &tot);  // (its barrier also publishes the edge slots)
+			if constexpr (!ONE_SCAN) {
+				i32 tot;
+				i32 pos = n_f + b.scan(nv, &tot);  // (its barrier also publishes the edge slots)
 #pragma unroll
-			for (i32 s = 0; s < S; ++s)
-				if (vend[s]) {
-					if (pos < cap.f) front_put(w, w.F0, w.FL0, pos, nk[s]);
-					++pos;
-				}
-			n_f += tot;
+				for (i32 s = 0; s < S; ++s)
+					if (vend[s]) {
+						if (pos < cap.f) front_put(w, w.F0, w.FL0, pos, nk[s]);
+						++pos;
+					}
+				n_f += tot;
+			} else {
+				(void)nv;
+			}
 		}
-		if (r > 0) {  // successors of round r - 1
+		if (r > 0) {  // successors (ONE_SCAN: and vertex ends) of round r - 1
+			if constexpr (ONE_SCAN) b.sync();  // publishes the edge slots of round r
 			u64 c[S + 2];  // c[0] = left neighbour, c[1..S] = pk, c[S + 1] = right neighbour
 			c[0] = dpp_u64<DPP_WAVE_SHR1, 0xf>(pk[S - 1]);
 			c[S + 1] = dpp_u64<DPP_WAVE_SHL1, 0xf>(pk[0]);
@@ -1596,8 +1620,24 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 				cs += (i32)sc[s].s0 + (i32)sc[s].s1 + (i32)sc[s].s2;
 				ct += (i32)sc[s].t0 + (i32)sc[s].t1 + (i32)sc[s].t2;
 			}
-			i32 tot, pt, tot_t;
-			i32 p = n_s + b.scan2(cs, ct, &pt, &tot, &tot_t);
+			i32 tot, pt, tot_t, p;
+			if constexpr (ONE_SCAN) {
+				i32 nvp = 0;
+#pragma unroll
+				for (i32 s = 0; s < S; ++s) nvp += pve[s];
+				i32 pq, tf;
+				i32 pos = n_f + b.scan3(nvp, cs, ct, &pq, &pt, &tf, &tot, &tot_t);
+#pragma unroll
+				for (i32 s = 0; s < S; ++s)
+					if (pve[s]) {
+						if (pos < cap.f) front_put(w, w.F0, w.FL0, pos, pk[s]);
+						++pos;
+					}
+				n_f += tf;
+				p = n_s + pq;
+			} else {
+				p = n_s + b.scan2(cs, ct, &pt, &tot, &tot_t);
+			}
 			if (n_s + tot > cap.m) ovf = true;
 			else {
 #pragma unroll
@@ -1623,7 +1663,7 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 			n_t += tot_t;
 		}
 #pragma unroll
-		for (i32 s = 0; s < S; ++s) pk[s] = nk[s], pvl[s] = nvl[s];
+		for (i32 s = 0; s < S; ++s) pk[s] = nk[s], pvl[s] = nvl[s], pve[s] = vend[s];
 	}
 	if (b.any(found)) return STEP_FOUND;
 	if (n_f > cap.f || ovf) return STEP_SPILL;  // (ovf is uniform: derived from block totals)
