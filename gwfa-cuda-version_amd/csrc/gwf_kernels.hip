@@ -119,6 +119,19 @@ __device__ __forceinline__ i32 wave_scan(i32 x, i32 *total) {
 	return v - x;
 }
 
+// the same in unsigned arithmetic (packed fields that may reach bit 31)
+__device__ __forceinline__ u32 wave_scan_u32(u32 x, u32 *total) {
+	u32 v = x;
+	v += (u32)dpp_i32<DPP_ROW_SHR(1), 0xf>(0, (i32)v);
+	v += (u32)dpp_i32<DPP_ROW_SHR(2), 0xf>(0, (i32)v);
+	v += (u32)dpp_i32<DPP_ROW_SHR(4), 0xf>(0, (i32)v);
+	v += (u32)dpp_i32<DPP_ROW_SHR(8), 0xf>(0, (i32)v);
+	v += (u32)dpp_i32<DPP_ROW_BCAST15, 0xa>(0, (i32)v);
+	v += (u32)dpp_i32<DPP_ROW_BCAST31, 0xc>(0, (i32)v);
+	*total = (u32)__builtin_amdgcn_readlane((i32)v, WAVE - 1);
+	return v - x;
+}
+
 __device__ __forceinline__ u64 umax64(u64 a, u64 b) { return a > b ? a : b; }
 
 // inclusive prefix max over the wave (u64)
@@ -193,19 +206,36 @@ struct Blk {
 		*tc = t >> 16;
 		return p & 0xffff;
 	}
-	// three exclusive scans in one for NW <= 2 (per-thread values below 8,
-	// block totals below 2^9, 2^10, 2^10, so the packed sum stays below 2^29):
-	// *pb / *pc receive the second and third, *ta, *tb, *tc the totals
+	// three exclusive scans in one, in unsigned arithmetic (per-thread values
+	// a <= 2, bb and c <= 6; block totals below 2^10, 2^11 and 2^11 for at
+	// most 256 threads): *pb / *pc receive the second and third, *ta, *tb,
+	// *tc the totals
 	__device__ __forceinline__ i32 scan3(i32 a, i32 bb, i32 c, i32 *pb, i32 *pc, i32 *ta, i32 *tb, i32 *tc) const {
-		static_assert(NW <= 2, "packed fields sized for at most 128 threads");
-		i32 t;
-		const i32 p = scan(a | bb << 9 | c << 19, &t);
-		*pb = (p >> 9) & 0x3ff;
-		*pc = p >> 19;
-		*ta = t & 0x1ff;
-		*tb = (t >> 9) & 0x3ff;
-		*tc = t >> 19;
-		return p & 0x1ff;
+		static_assert(NW <= 4, "packed fields sized for at most 256 threads");
+		u32 wt;
+		u32 p = wave_scan_u32((u32)a | (u32)bb << 10 | (u32)c << 21, &wt);
+		u32 t = wt;
+		if constexpr (NW > 1) {
+			li32 *r = red + par * NW;
+			par ^= 1;
+			if (lane_id() == 0) r[wid()] = (i32)wt;
+			sync();
+			u32 pre = 0;
+			t = 0;
+#pragma unroll
+			for (int w = 0; w < NW; ++w) {
+				const u32 v = (u32)r[w];
+				pre += w < (int)wid() ? v : 0u;
+				t += v;
+			}
+			p += pre;
+		}
+		*pb = (i32)((p >> 10) & 0x7ff);
+		*pc = (i32)(p >> 21);
+		*ta = (i32)(t & 0x3ff);
+		*tb = (i32)((t >> 10) & 0x7ff);
+		*tc = (i32)(t >> 21);
+		return (i32)(p & 0x3ff);
 	}
 	// compaction slot of a predicate; *n advances by the block count
 	__device__ __forceinline__ i32 slot(bool pred, i32 *n) const {
@@ -1511,11 +1541,10 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 	i32 n_f = 0, n_s = 0, n_t = 0;
 	u64 pk[S];   // the previous round's extended states (EMPTY_KEY past the end)
 	i32 pvl[S];  // and their vertex lengths
-	// With at most two waves a round's vertex ends go to the frontier one
-	// round later, with its successors: one scan places all three lists, and
-	// a plain barrier publishes the edge slots before the successors
-	constexpr bool ONE_SCAN = NW <= 2;
-	bool pve[S];  // the previous round's vertex ends (ONE_SCAN)
+	// A round's vertex ends go to the frontier one round later, with its
+	// successors: one scan places all three lists, and a plain barrier
+	// publishes the edge slots before the successors
+	bool pve[S];  // the previous round's vertex ends
 #pragma unroll
 	for (i32 s = 0; s < S; ++s) pk[s] = EMPTY_KEY, pvl[s] = 0, pve[s] = false;
 	for (i32 r = 0; r <= R; ++r) {
@@ -1564,7 +1593,6 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 #pragma unroll
 			for (i32 s = 0; s < S; ++s)  // every thread calls it S times (it is a wave collective)
 				kk[s] = extend_wave(more[s], g.seq, vi[s].x, q, key_d(key[s]), kk[s], mk8[s], &chars);
-			i32 nv = 0;
 #pragma unroll
 			for (i32 s = 0; s < S; ++s) {
 				const bool act = key[s] != EMPTY_KEY;
@@ -1583,26 +1611,12 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 				// vertex ends go to the closure; with v1 >= 0 that includes query
 				// ends, which continue by deletion into the next vertices
 				vend[s] = act && !fnd && k == vl - 1;
-				nv += vend[s];
 			}
 			if (lane == 0) edgeF[(r % 3) * NW + wv] = nk[0];
 			if (lane == WAVE - 1) edgeL[(r % 3) * NW + wv] = nk[S - 1];
-			if constexpr (!ONE_SCAN) {
-				i32 tot;
-				i32 pos = n_f + b.scan(nv, &tot);  // (its barrier also publishes the edge slots)
-#pragma unroll
-				for (i32 s = 0; s < S; ++s)
-					if (vend[s]) {
-						if (pos < cap.f) front_put(w, w.F0, w.FL0, pos, nk[s]);
-						++pos;
-					}
-				n_f += tot;
-			} else {
-				(void)nv;
-			}
 		}
-		if (r > 0) {  // successors (ONE_SCAN: and vertex ends) of round r - 1
-			if constexpr (ONE_SCAN) b.sync();  // publishes the edge slots of round r
+		if (r > 0) {  // successors and vertex ends of round r - 1
+			b.sync();  // publishes the edge slots of round r
 			u64 c[S + 2];  // c[0] = left neighbour, c[1..S] = pk, c[S + 1] = right neighbour
 			c[0] = dpp_u64<DPP_WAVE_SHR1, 0xf>(pk[S - 1]);
 			c[S + 1] = dpp_u64<DPP_WAVE_SHL1, 0xf>(pk[0]);
@@ -1621,7 +1635,7 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 				ct += (i32)sc[s].t0 + (i32)sc[s].t1 + (i32)sc[s].t2;
 			}
 			i32 tot, pt, tot_t, p;
-			if constexpr (ONE_SCAN) {
+			{
 				i32 nvp = 0;
 #pragma unroll
 				for (i32 s = 0; s < S; ++s) nvp += pve[s];
@@ -1635,8 +1649,6 @@ __device__ __forceinline__ int gwf_step(const DGraph &g, const uint8_t *q, i32 q
 					}
 				n_f += tf;
 				p = n_s + pq;
-			} else {
-				p = n_s + b.scan2(cs, ct, &pt, &tot, &tot_t);
 			}
 			if (n_s + tot > cap.m) ovf = true;
 			else {
